@@ -1,0 +1,248 @@
+/*
+ * mcpi_hip.h -- C ABI of libmcpi_hip.so, an MI355X (gfx950) implementation of
+ * the mcpi matrix-free Fock-space hot path: perfect hashing of bosonic
+ * occupation vectors, single/double jump maps, H.v, one- and two-body
+ * density matrices, and the Lanczos ground state.
+ *
+ * Two layers:
+ *
+ *  1. Reference-shaped entry points (same names, argument order and meaning
+ *     as andriati-alex/mcpi's headers; host pointers; synchronous; results in
+ *     host memory).  A program written against the reference includes this
+ *     header instead of configurationsMap.h / hamiltonianMatrix.h /
+ *     onebodyMatrix.h / twobodyMatrix.h, links -lmcpi_hip, and runs the hot
+ *     loops on the GPU.  Like the reference they print a message and exit()
+ *     on the reference's INT32 overflow guards.
+ *
+ *  2. The extended API (prefix mcpi_hip_): device-resident bases and
+ *     Hamiltonians as opaque handles, int64 offsets (bases beyond the
+ *     reference's INT32 limits), device pointers and streams, batched
+ *     Lanczos and the Hellmann-Feynman adjoint.  Every call returns an
+ *     mcpi_status (0 = success) and never exits; mcpi_hip_last_error() gives
+ *     the message of the last failure on the calling thread.
+ *
+ * Layouts (shared by both layers; "first index fastest"):
+ *   IF[i*M + j]                       occupation of orbital j in state i
+ *   NCmat[n + (N+1)*m]                NC(n, m) = C(n+m-1, n); column 0 is 0
+ *   Map[i + k*nc + l*M*nc]            index of v_i - e_k + e_l, -1 if v_k = 0
+ *   MapOT[strideOT[i] + c*M*M + l + q*M]  v_i - 2e_k + e_l + e_q, c = ordinal
+ *                                     of k among orbitals with v >= 2
+ *   MapTT[strideTT[i] + c*M*M + l + q*M]  v_i - e_k - e_s + e_l + e_q, c =
+ *                                     lexicographic ordinal of the pair k<s
+ *                                     among occupied pairs
+ *   Ho (flat) [k + M*l]               coefficient of a*_k a_l
+ *   Hint      [k + s*M + q*M^2 + l*M^3]  coefficient of a*_k a*_s a_q a_l
+ *   rho1 (flat) [k + M*l] = <a*_k a_l>;  rho2 [k + sM + qM^2 + lM^3] =
+ *                                     <a*_k a*_s a_q a_l>
+ * Complex numbers are two doubles (re, im): C99 double _Complex, C++
+ * std::complex<double> and hipDoubleComplex all have this layout.
+ * H = sum Ho[k,l] a*_k a_l + 1/2 sum Hint[k,s,q,l] a*_k a*_s a_q a_l, with the
+ * reference's folded two-body rules (exact for a Hint with full bosonic
+ * symmetry Hint[k,s,q,l] = Hint[s,k,q,l] = Hint[k,s,l,q] = conj(Hint[q,l,k,s])).
+ */
+#ifndef MCPI_HIP_H
+#define MCPI_HIP_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+#include <complex>
+typedef std::complex<double> mcpi_cplx;
+extern "C" {
+#else
+#include <complex.h>
+typedef double _Complex mcpi_cplx;
+#endif
+
+/* Reference typedefs, configurationsMap.h:27-40 */
+typedef double *Rarray;
+typedef mcpi_cplx *Carray;
+typedef mcpi_cplx **Cmatrix;
+typedef int *Iarray;
+
+/* ======================================================================
+ * 1. Reference-shaped entry points
+ * ====================================================================== */
+
+/* configurationsMap.h:145  NC(N, M) = C(N+M-1, N); exits on INT32 overflow */
+int NC(int N, int M);
+/* configurationsMap.h:206  malloc'ed (N+1)*(M+1) table, free() it */
+Iarray setupNCmat(int N, int M);
+/* configurationsMap.h:238  unrank k into v[M] */
+void IndexToFock(int k, int N, int M, Iarray v);
+/* configurationsMap.h:273  rank of v */
+int FockToIndex(int N, int M, Iarray NCmat, Iarray v);
+/* configurationsMap.h:309  malloc'ed nc*M IF table (built on the GPU) */
+Iarray setupFocks(int N, int M);
+/* configurationsMap.h:348  malloc'ed nc*M*M single-jump map (built on the GPU) */
+Iarray OneOneMap(int N, int M, Iarray NCmat, Iarray IF);
+/* configurationsMap.h:478  malloc'ed two-orbital double-jump map; fills strideC[nc] */
+Iarray TwoTwoMap(int N, int M, Iarray NCmat, Iarray IF, Iarray strideC);
+/* configurationsMap.h:668  malloc'ed same-orbital double-jump map; fills strideC[nc] */
+Iarray OneTwoMap(int N, int M, Iarray NCmat, Iarray IF, Iarray strideC);
+
+/* hamiltonianMatrix.h:1431  out = H C with all maps (host arrays; H.v on the GPU) */
+void applyHconf_omp(int N, int M, Iarray Map, Iarray MapOT, Iarray MapTT,
+                    Iarray strideOT, Iarray strideTT, Iarray IF, Carray C,
+                    Cmatrix Ho, Carray Hint, Carray out);
+/* hamiltonianMatrix.h:891  same computation as applyHconf_omp */
+void applyHconf_XX(int N, int M, Iarray Map, Iarray MapOT, Iarray MapTT,
+                   Iarray strideOT, Iarray strideTT, Iarray IF, Carray C,
+                   Cmatrix Ho, Carray Hint, Carray out);
+/* hamiltonianMatrix.h:31  hash-only H.v (double jumps ranked on the fly) */
+void applyHconf(int N, int M, Iarray NCmat, Iarray IF, Carray C, Cmatrix Ho,
+                Carray Hint, Carray out);
+/* onebodyMatrix.h:190  rho[k][l] = <a*_k a_l> */
+void OBrho_XM(int N, int M, Iarray Map, Iarray NCmat, Iarray IF, Carray C,
+              Cmatrix rho);
+/* twobodyMatrix.h:991  rho[k + sM + qM^2 + lM^3] = <a*_k a*_s a_q a_l> */
+void TBrho_XX(int N, int M, Iarray Map, Iarray MapOT, Iarray MapTT,
+              Iarray strideOT, Iarray strideTT, Iarray NCmat, Iarray IF,
+              Carray C, Carray rho);
+/* groundStateLieb.c:217  Lanczos with full reorthogonalization; lvec[lm][nc],
+ * lvec[0] = normalized start vector.  Returns the number of steps kept. */
+int lanczos(int lm, int Npar, int Morb, Iarray IF, Iarray strideOT,
+            Iarray strideTT, Iarray Map, Iarray MapOT, Iarray MapTT,
+            Cmatrix Ho, Carray Hint, Carray diag, Carray offdiag, Cmatrix lvec);
+/* groundStateLieb.c:348  lowest eigenpair; C in: start vector, out: Ritz vector */
+double ground(int Niter, int Npar, int Morb, Iarray IF, Iarray strideOT,
+              Iarray strideTT, Iarray Map, Iarray MapOT, Iarray MapTT,
+              Cmatrix Ho, Carray Hint, Carray C);
+/* groundStateLieb.c:73, :89  (the C++ view of the complex return value has
+ * the same x86-64 ABI as C's double _Complex: two doubles in xmm0/xmm1) */
+#if defined(__cplusplus) && defined(__clang__)
+#pragma clang diagnostic push
+#pragma clang diagnostic ignored "-Wreturn-type-c-linkage"
+#endif
+mcpi_cplx carrDot(int n, Carray v1, Carray v2);
+#if defined(__cplusplus) && defined(__clang__)
+#pragma clang diagnostic pop
+#endif
+double carrMod(int n, Carray v);
+/* groundStateLieb.c:129, :159  Lieb-Liniger model on a ring of length L */
+void setupHo(int Morb, double L, Cmatrix Ho);
+void setupHint(int Morb, double g, Carray Hint);
+
+/* ======================================================================
+ * 2. Extended API
+ * ====================================================================== */
+
+typedef int mcpi_status;
+#define MCPI_OK 0
+#define MCPI_ERR_ARG 1       /* invalid argument or size beyond a limit   */
+#define MCPI_ERR_HIP 2       /* HIP runtime error (message has details)   */
+#define MCPI_ERR_ALLOC 3     /* device or host allocation failed          */
+#define MCPI_ERR_STATE 4     /* handle in the wrong state for the call    */
+
+/* storage modes of a basis (the reference's memory-for-time ladder,
+ * hamiltonianMatrix.h:31 / :431 / :891) */
+#define MCPI_MAPS_NONE 0     /* IF only: every jump ranked on the fly       */
+#define MCPI_MAPS_SINGLE 1   /* IF + Map: double jumps ranked on the fly    */
+#define MCPI_MAPS_FULL 2     /* IF + Map + MapOT + MapTT                    */
+
+typedef struct mcpi_basis_s *mcpi_basis;
+typedef struct mcpi_ham_s *mcpi_ham;
+
+typedef struct {
+    int N, M, maps;
+    int64_t nc;              /* number of configurations                    */
+    int64_t map_len;         /* nc*M*M or 0                                 */
+    int64_t ot_len, tt_len;  /* |MapOT|, |MapTT| in ints (0 if not stored)  */
+    int64_t device_bytes;    /* device memory held by the basis             */
+} mcpi_basis_info;
+
+const char *mcpi_hip_last_error(void);
+const char *mcpi_hip_version(void);
+
+/* device memory and synchronization helpers (HIP runtime of the library) */
+mcpi_status mcpi_hip_malloc(void **dptr, size_t bytes);
+mcpi_status mcpi_hip_free(void *dptr);
+mcpi_status mcpi_hip_memcpy_h2d(void *dst, const void *src, size_t bytes);
+mcpi_status mcpi_hip_memcpy_d2h(void *dst, const void *src, size_t bytes);
+mcpi_status mcpi_hip_memcpy_d2d(void *dst, const void *src, size_t bytes);
+mcpi_status mcpi_hip_memset(void *dptr, int value, size_t bytes);
+mcpi_status mcpi_hip_synchronize(void);
+mcpi_status mcpi_hip_device_info(char *name, size_t name_len, int *cu_count,
+                                 size_t *total_mem);
+
+/* Build a device-resident basis (tables built on the GPU, int64 offsets). */
+mcpi_status mcpi_hip_basis_create(int N, int M, int maps, mcpi_basis *out);
+mcpi_status mcpi_hip_basis_destroy(mcpi_basis b);
+mcpi_status mcpi_hip_basis_info(mcpi_basis b, mcpi_basis_info *info);
+/* copies of the tables to host memory (sizes from mcpi_basis_info) */
+mcpi_status mcpi_hip_basis_copy_ncmat(mcpi_basis b, int32_t *NCmat);
+mcpi_status mcpi_hip_basis_copy_if(mcpi_basis b, int32_t *IF);
+mcpi_status mcpi_hip_basis_copy_map(mcpi_basis b, int32_t *Map);
+mcpi_status mcpi_hip_basis_copy_ot(mcpi_basis b, int64_t *strideOT, int32_t *MapOT);
+mcpi_status mcpi_hip_basis_copy_tt(mcpi_basis b, int64_t *strideTT, int32_t *MapTT);
+/* count entries of one table from offset (which: 0 Map, 1 MapOT, 2 MapTT,
+ * 3 IF, 4 strideOT, 5 strideTT; strides are int64, the others int32) */
+mcpi_status mcpi_hip_basis_copy_range(mcpi_basis b, int which, int64_t offset,
+                                      int64_t count, void *dst);
+/* rank / unrank of many states on the GPU: v is n*M ints (host) */
+mcpi_status mcpi_hip_basis_rank(mcpi_basis b, int64_t n, const int32_t *v, int64_t *idx);
+mcpi_status mcpi_hip_basis_unrank(mcpi_basis b, int64_t n, const int64_t *idx, int32_t *v);
+
+/* A Hamiltonian on a basis: Ho (M*M, flat k + M*l) and Hint (M^4), host
+ * arrays, copied and pre-folded into the per-rule coefficient tables. */
+mcpi_status mcpi_hip_ham_create(mcpi_basis b, const mcpi_cplx *Ho,
+                                const mcpi_cplx *Hint, mcpi_ham *out);
+mcpi_status mcpi_hip_ham_destroy(mcpi_ham h);
+/* Terms whose folded coefficient is exactly zero are skipped (result-neutral
+ * for finite inputs; default on).  Off = every term of the storage mode is
+ * evaluated and every map entry read, as the reference does. */
+mcpi_status mcpi_hip_ham_set_skip_zero(mcpi_ham h, int on);
+/* kernel schedule: 0 = auto (tuned), 1 = thread-per-row, 2 = tiled tasks */
+mcpi_status mcpi_hip_ham_set_schedule(mcpi_ham h, int schedule);
+
+/* out = H C on device pointers (nc complex each), on stream (NULL = default) */
+mcpi_status mcpi_hip_applyH(mcpi_ham h, const mcpi_cplx *C_dev, mcpi_cplx *out_dev,
+                            void *stream);
+/* CUDA-kernel-shaped H.v on raw device tables (cuda/hamiltonianMatrix.cuh:61):
+ * every pointer is device memory, Ho flat k + M*l, int64 strides. */
+mcpi_status mcpi_hip_applyHconf_dev(int N, int M, const int32_t *Map,
+                                    const int32_t *MapOT, const int32_t *MapTT,
+                                    const int64_t *strideOT, const int64_t *strideTT,
+                                    const int32_t *IF, const mcpi_cplx *C,
+                                    const mcpi_cplx *Ho, const mcpi_cplx *Hint,
+                                    mcpi_cplx *out, void *stream);
+/* average duration (ms) of the H.v kernels of the last n calls of
+ * mcpi_hip_applyH_timed, measured with HIP events on the launch stream */
+mcpi_status mcpi_hip_applyH_timed(mcpi_ham h, const mcpi_cplx *C_dev,
+                                  mcpi_cplx *out_dev, int reps, void *stream,
+                                  double *avg_ms);
+
+/* density matrices of C (device vector): rho1 M*M, rho2 M^4, host outputs */
+mcpi_status mcpi_hip_rho1(mcpi_basis b, const mcpi_cplx *C_dev, mcpi_cplx *rho1);
+mcpi_status mcpi_hip_rho2(mcpi_basis b, const mcpi_cplx *C_dev, mcpi_cplx *rho2);
+/* Hellmann-Feynman adjoint at a normalized eigenvector C: dE/dHo = rho1,
+ * dE/dHint = rho2 / 2 (host outputs, flat layouts above) */
+mcpi_status mcpi_hip_vjp(mcpi_basis b, const mcpi_cplx *C_dev, mcpi_cplx *dE_dHo,
+                         mcpi_cplx *dE_dHint);
+
+/* Lanczos on the device.  C0_dev: normalized start vector (nc).  lvec_dev:
+ * lm*nc device buffer for the basis (row t = v_t).  diag/offdiag: host, lm
+ * each.  *steps = size of the tridiagonal actually built. */
+mcpi_status mcpi_hip_lanczos(mcpi_ham h, int lm, const mcpi_cplx *C0_dev,
+                             mcpi_cplx *lvec_dev, mcpi_cplx *diag, mcpi_cplx *offdiag,
+                             int *steps);
+/* Ground state: Lanczos of length lm from C0_dev, host tridiagonal solve,
+ * Ritz vector into C_out_dev (may alias C0_dev).  *E0 = lowest eigenvalue. */
+mcpi_status mcpi_hip_ground(mcpi_ham h, int lm, const mcpi_cplx *C0_dev,
+                            mcpi_cplx *C_out_dev, double *E0, int *steps);
+/* lowest eigenpair of the real symmetric tridiagonal (d[n], e[n-1]) on the
+ * host: implicit QL with Wilkinson shifts; z (n) may be NULL */
+mcpi_status mcpi_hip_tridiag_lowest(int n, const double *d, const double *e,
+                                    double *lambda, double *z);
+
+/* Batched Lanczos: n instances, each a Hamiltonian handle (instances may
+ * share a basis) and a device start vector; lm steps each; E0[n] out. */
+mcpi_status mcpi_hip_lanczos_batched(int n, const mcpi_ham *hams,
+                                     const mcpi_cplx *const *C0_dev, int lm,
+                                     double *E0, int *steps);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* MCPI_HIP_H */

@@ -1,0 +1,303 @@
+// H.v (actH) on the GPU.
+//
+// Reference: applyHconf_omp, hamiltonianMatrix.h:1431-1970 (rules 1-9 and
+// the one-body term, out = w + 0.5 z; identical maths in applyHconf_XX
+// 891-1425, applyHconf_X 431-887 and the hash-only applyHconf 31-425).
+//
+// The rules are folded once per Hamiltonian into coefficient tables indexed
+// by the target a row gathers from (build_coefficients below):
+//   * target i itself        (1-body diagonal, rules 1 and 2)
+//   * single jump k -> l     (1-body off-diagonal, rules 4, 5 and 8 all
+//                             gather C[Map[i,k,l]]: one gather instead of up
+//                             to 2 + #occupied)
+//   * same-orbital double    (rules 3 and 6, OT chunk entry q + l*M, q <= l)
+//   * two-orbital double     (rules 7 and 9, TT chunk entry q + l*M, q <= l)
+// so every gathered C[j] is multiplied by one complex coefficient times a
+// product of square roots of occupations.  Coefficients are uniform across
+// a wavefront (they depend only on the orbitals), occupations per lane.
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <cstring>
+#include <vector>
+
+#include "mcpi_internal.h"
+
+namespace mcpi {
+
+// --------------------------------------------------------- coefficients --
+
+mcpi_status build_coefficients(int M, const cplx *Ho, const cplx *Hint, HamCoefDev *c, void **block)
+{
+    const int64_t M2 = (int64_t)M * M, M3 = M2 * M;
+    const int P = M * (M - 1) / 2;
+    auto H4 = [&](int k, int s, int q, int l) -> cplx {
+        return Hint[k + M * (s + (int64_t)M * (q + (int64_t)M * l))];
+    };
+    auto H2 = [&](int k, int l) -> cplx { return Ho[k + M * l]; };
+    std::vector<cplx> dk(M), d1(M), d2(M2), A0(M2), A(M3), ot(M * M2), tt((size_t)std::max(P, 1) * M2);
+    std::vector<uint8_t> nz1(M2), nzo(M * M2), nzt((size_t)std::max(P, 1) * M2);
+    auto nzc = [](cplx z) { return z.re != 0.0 || z.im != 0.0; };
+    for (int k = 0; k < M; k++) {
+        dk[k] = H2(k, k);                                 // w: Ho[k][k] v_k
+        d1[k] = cscale(H4(k, k, k, k), 0.5);              // rule 1 (x 1/2)
+        for (int s = k + 1; s < M; s++) d2[k + M * s] = cscale(H4(k, s, k, s), 2.0);  // rule 2: 4 x 1/2
+    }
+    for (int k = 0; k < M; k++)
+        for (int l = 0; l < M; l++) {
+            if (l == k) continue;
+            int kl = k + M * l;
+            // 1-body Ho[k][l]; rule 4: 2 Hint[kkkl] (v_k - 1) x 1/2; rule 5 (s = l):
+            // 2 Hint[klll] v_l x 1/2; rule 8: 4 Hint[kssl] v_s x 1/2 for s not in {k,l}
+            cplx h4 = H4(k, k, k, l), h5 = H4(k, l, l, l);
+            A0[kl] = cmake(H2(k, l).re - h4.re, H2(k, l).im - h4.im);
+            for (int s = 0; s < M; s++) {
+                cplx a;
+                if (s == k) a = h4;
+                else if (s == l) a = h5;
+                else a = cscale(H4(k, s, s, l), 2.0);
+                A[(int64_t)kl * M + s] = a;
+                if (nzc(a)) nz1[kl] = 1;
+            }
+            if (nzc(A0[kl])) nz1[kl] = 1;
+        }
+    for (int k = 0; k < M; k++)
+        for (int l = 0; l < M; l++)
+            for (int q = 0; q <= l; q++) {
+                if (q == k || l == k) continue;
+                int64_t e = q + (int64_t)M * l;
+                // rule 3 (q == l): Hint[kkqq] x 1/2; rule 6 (q < l): 2 Hint[kkql] x 1/2
+                cplx a = (q == l) ? cscale(H4(k, k, q, q), 0.5) : H4(k, k, q, l);
+                ot[k * M2 + e] = a;
+                nzo[k * M2 + e] = nzc(a);
+            }
+    for (int k = 0; k < M; k++)
+        for (int s = k + 1; s < M; s++) {
+            int p = pair_index(k, s, M);
+            for (int l = 0; l < M; l++)
+                for (int q = 0; q <= l; q++) {
+                    if (q == k || q == s || l == k || l == s) continue;
+                    int64_t e = q + (int64_t)M * l;
+                    // rule 7 (q == l): 2 Hint[ksqq] x 1/2; rule 9 (q < l): 4 Hint[ksql] x 1/2
+                    cplx a = (q == l) ? H4(k, s, q, q) : cscale(H4(k, s, q, l), 2.0);
+                    tt[p * M2 + e] = a;
+                    nzt[p * M2 + e] = nzc(a);
+                }
+        }
+    // one device block: complex tables first (16-byte aligned), then masks
+    size_t nC = dk.size() + d1.size() + d2.size() + A0.size() + A.size() + ot.size() + tt.size();
+    size_t nB = nz1.size() + nzo.size() + nzt.size();
+    size_t bytes = nC * sizeof(cplx) + nB + 64;
+    char *dev = nullptr;
+    MCPI_HIP(hipMalloc(&dev, bytes));
+    std::vector<char> host(bytes, 0);
+    size_t off = 0;
+    auto put = [&](const void *src, size_t n) { memcpy(host.data() + off, src, n); size_t o = off; off += n; return o; };
+    size_t o_dk = put(dk.data(), dk.size() * sizeof(cplx));
+    size_t o_d1 = put(d1.data(), d1.size() * sizeof(cplx));
+    size_t o_d2 = put(d2.data(), d2.size() * sizeof(cplx));
+    size_t o_A0 = put(A0.data(), A0.size() * sizeof(cplx));
+    size_t o_A = put(A.data(), A.size() * sizeof(cplx));
+    size_t o_ot = put(ot.data(), ot.size() * sizeof(cplx));
+    size_t o_tt = put(tt.data(), tt.size() * sizeof(cplx));
+    size_t o_n1 = put(nz1.data(), nz1.size());
+    size_t o_no = put(nzo.data(), nzo.size());
+    size_t o_nt = put(nzt.data(), nzt.size());
+    hipError_t e = hipMemcpy(dev, host.data(), bytes, hipMemcpyHostToDevice);
+    if (e != hipSuccess) {
+        (void)hipFree(dev);
+        return hip_fail(e, "coefficient upload", __FILE__, __LINE__);
+    }
+    c->dk = (cplx *)(dev + o_dk);
+    c->d1 = (cplx *)(dev + o_d1);
+    c->d2 = (cplx *)(dev + o_d2);
+    c->A0 = (cplx *)(dev + o_A0);
+    c->A = (cplx *)(dev + o_A);
+    c->ot = (cplx *)(dev + o_ot);
+    c->tt = (cplx *)(dev + o_tt);
+    c->nz_1j = (uint8_t *)(dev + o_n1);
+    c->nz_ot = (uint8_t *)(dev + o_no);
+    c->nz_tt = (uint8_t *)(dev + o_nt);
+    *block = dev;
+    return MCPI_OK;
+}
+
+// ------------------------------------------------------------- kernels --
+
+__device__ inline cplx ldc(const cplx *p)
+{
+    double2 t = *reinterpret_cast<const double2 *>(p);
+    return cmake(t.x, t.y);
+}
+
+// rank of v - e_a - e_b + e_c + e_d (b, d may be -1) from the row's prefix
+// counts held in LDS (column layout Ps[j*T + t]) and the table B.
+__device__ inline int64_t rank_lds(const uint8_t *Ps, int T, int t, const int32_t *B, int N, int M,
+                                   int64_t nc, int a, int b, int c, int d)
+{
+    int64_t s = 0;
+    for (int j = 1; j < M; j++) {
+        int p = (int)Ps[j * T + t] - (a < j) - (b >= 0 && b < j) + (c < j) + (d >= 0 && d < j);
+        s += B[j * (N + 1) + p];
+    }
+    return nc - 1 - s;
+}
+
+// Thread-per-row H.v for all storage modes.
+//   MODE 2: Map + MapOT + MapTT (applyHconf_omp / _XX)
+//   MODE 1: Map, double jumps ranked (applyHconf_X ranks TT only; here both)
+//   MODE 0: every jump ranked (applyHconf)
+// Lanes are consecutive rows, so a term's map reads are coalesced (Map) and
+// its gathers C[j] land close together; the loops over orbitals are uniform.
+template <int MODE>
+__global__ __launch_bounds__(256) void k_acth_rows(BasisDev b, HamCoefDev c, const cplx *__restrict__ C,
+                                                   cplx *__restrict__ out, int skip_zero)
+{
+    extern __shared__ double smem_d[];
+    const int M = b.M, N = b.N, T = blockDim.x, t = threadIdx.x;
+    const int64_t nc = b.nc, MM = (int64_t)M * M;
+    double *sq = smem_d;                                    // sqrt(n), n = 0..N+2
+    int32_t *Bt = (int32_t *)(sq + N + 3);                  // M*(N+1)
+    uint8_t *Vs = (uint8_t *)(Bt + M * (N + 1));            // M*T
+    uint8_t *Ps = Vs + M * T;                               // M*T (prefix counts)
+    for (int x = t; x < N + 3; x += T) sq[x] = sqrt((double)x);
+    if (MODE < 2)
+        for (int x = t; x < M * (N + 1); x += T) Bt[x] = b.B[x];
+    const int64_t i = (int64_t)blockIdx.x * T + t;
+    const bool valid = i < nc;
+    {
+        int acc = 0;
+        for (int j = 0; j < M; j++) {
+            int x = valid ? b.IF[i * M + j] : 0;
+            Vs[j * T + t] = (uint8_t)x;
+            Ps[j * T + t] = (uint8_t)acc;
+            acc += x;
+        }
+    }
+    __syncthreads();
+    if (!valid) return;
+#define V(j) ((int)Vs[(j) * T + t])
+    const cplx ci = ldc(C + i);
+
+    // diagonal: 1-body diagonal, rules 1 and 2
+    cplx dsum = cmake(0, 0);
+    for (int k = 0; k < M; k++) {
+        int vk = V(k);
+        if (vk == 0) continue;
+        dsum = cfma(c.dk[k], cmake((double)vk, 0), dsum);
+        dsum = cfma(c.d1[k], cmake((double)vk * (vk - 1), 0), dsum);
+        for (int s = k + 1; s < M; s++) dsum = cfma(c.d2[k + M * s], cmake((double)vk * V(s), 0), dsum);
+    }
+    cplx acc = cmake(0, 0);
+    acc = cfma(dsum, ci, acc);
+
+    // single jumps k -> l: 1-body off-diagonal, rules 4, 5, 8
+    for (int k = 0; k < M; k++) {
+        const int vk = V(k);
+        for (int l = 0; l < M; l++) {
+            if (l == k) continue;
+            const int kl = k + M * l;
+            if (skip_zero && !c.nz_1j[kl]) continue;
+            if (vk < 1) continue;
+            cplx h = c.A0[kl];
+            const cplx *Ak = c.A + (int64_t)kl * M;
+            for (int s = 0; s < M; s++) {
+                double vs = (double)V(s);
+                h.re = fma(vs, Ak[s].re, h.re);
+                h.im = fma(vs, Ak[s].im, h.im);
+            }
+            int64_t j;
+            if (MODE >= 1) j = b.Map[i + (int64_t)kl * nc];
+            else j = rank_lds(Ps, T, t, Bt, N, M, nc, k, -1, l, -1);
+            double f = sq[vk] * sq[V(l) + 1];
+            acc = cfma(h, cscale(ldc(C + j), f), acc);
+        }
+    }
+
+    // same-orbital double jumps: rules 3 and 6
+    int64_t cot = 0;
+    for (int k = 0; k < M; k++) {
+        const int vk = V(k);
+        if (vk < 2) continue;
+        const double fk = sq[vk] * sq[vk - 1];
+        const int64_t base = (MODE == 2) ? b.strideOT[i] + cot * MM : 0;
+        cot++;
+        for (int l = 0; l < M; l++) {
+            if (l == k) continue;
+            for (int q = 0; q <= l; q++) {
+                if (q == k) continue;
+                const int e = q + M * l;
+                if (skip_zero && !c.nz_ot[k * MM + e]) continue;
+                double x = (q == l) ? sq[V(q) + 1] * sq[V(q) + 2] : sq[V(q) + 1] * sq[V(l) + 1];
+                int64_t j;
+                if (MODE == 2) j = b.MapOT[base + e];
+                else j = rank_lds(Ps, T, t, Bt, N, M, nc, k, k, q, l);
+                acc = cfma(c.ot[k * MM + e], cscale(ldc(C + j), fk * x), acc);
+            }
+        }
+    }
+
+    // two-orbital double jumps: rules 7 and 9
+    int64_t ctt = 0;
+    int p = 0;
+    for (int k = 0; k < M; k++) {
+        const int vk = V(k);
+        for (int s = k + 1; s < M; s++, p++) {
+            const int vs = V(s);
+            if (vk < 1 || vs < 1) continue;
+            const double fks = sq[vk] * sq[vs];
+            const int64_t base = (MODE == 2) ? b.strideTT[i] + ctt * MM : 0;
+            ctt++;
+            const cplx *cp = c.tt + (int64_t)p * MM;
+            const uint8_t *np = c.nz_tt + (int64_t)p * MM;
+            for (int l = 0; l < M; l++) {
+                if (l == k || l == s) continue;
+                for (int q = 0; q <= l; q++) {
+                    if (q == k || q == s) continue;
+                    const int e = q + M * l;
+                    if (skip_zero && !np[e]) continue;
+                    double x = (q == l) ? sq[V(q) + 1] * sq[V(q) + 2] : sq[V(q) + 1] * sq[V(l) + 1];
+                    int64_t j;
+                    if (MODE == 2) j = b.MapTT[base + e];
+                    else j = rank_lds(Ps, T, t, Bt, N, M, nc, k, s, q, l);
+                    acc = cfma(cp[e], cscale(ldc(C + j), fks * x), acc);
+                }
+            }
+        }
+    }
+#undef V
+    double2 r;
+    r.x = acc.re;
+    r.y = acc.im;
+    *reinterpret_cast<double2 *>(out + i) = r;
+}
+
+static size_t rows_smem(int N, int M, int T)
+{
+    return sizeof(double) * (N + 3) + sizeof(int32_t) * M * (N + 1) + 2 * (size_t)M * T + 16;
+}
+
+mcpi_status launch_apply_raw(const BasisDev &b, int maps, const HamCoefDev &c, int skip_zero,
+                             int schedule, const cplx *C, cplx *out, hipStream_t st)
+{
+    (void)schedule;
+    const int T = 256;
+    dim3 grid((unsigned)((b.nc + T - 1) / T));
+    size_t sm = rows_smem(b.N, b.M, T);
+    if (maps == MCPI_MAPS_FULL)
+        hipLaunchKernelGGL(k_acth_rows<2>, grid, dim3(T), sm, st, b, c, C, out, skip_zero);
+    else if (maps == MCPI_MAPS_SINGLE)
+        hipLaunchKernelGGL(k_acth_rows<1>, grid, dim3(T), sm, st, b, c, C, out, skip_zero);
+    else
+        hipLaunchKernelGGL(k_acth_rows<0>, grid, dim3(T), sm, st, b, c, C, out, skip_zero);
+    MCPI_HIP(hipGetLastError());
+    return MCPI_OK;
+}
+
+mcpi_status launch_apply(mcpi_ham h, const cplx *C, cplx *out, hipStream_t st)
+{
+    return launch_apply_raw(h->basis->d, h->basis->maps, h->c, h->skip_zero,
+                            h->schedule ? h->schedule : h->tuned_schedule, C, out, st);
+}
+
+}  // namespace mcpi

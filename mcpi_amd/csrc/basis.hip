@@ -1,0 +1,659 @@
+// Device-resident Fock bases: hashing tables and jump maps built on the GPU.
+//
+// Reference behaviour (andriati-alex/mcpi, configurationsMap.h):
+//   NC / setupNCmat        145-234
+//   IndexToFock (unrank)   238-269
+//   FockToIndex (rank)     273-305
+//   setupFocks (IF)        309-344
+//   OneOneMap              348-417
+//   allocTwoTwoMap / TwoTwoMap   421-613
+//   allocOneTwoMap / OneTwoMap   617-762
+// The values are bit-identical to the reference; offsets are int64 so bases
+// whose maps exceed INT32 (the reference refuses them) can be built.
+//
+// Ranking uses the closed form of the FockToIndex loop.  With prefix counts
+// P_j = sum_{o<j} v_o the loop adds, for every orbital i >= 1, the sum of
+// NC(n, i) over n in (P_i, P_{i+1}]; summing the hockey-stick identity gives
+//     rank(v) = nc - 1 - sum_{j=1}^{M-1} C(P_j + j - 1, j),
+// so a jump that moves particles only changes the P_j between the orbitals
+// involved, and the rank of any target is one pass over j with table
+// lookups B[j][p] = C(p + j - 1, j).
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cstdio>
+#include <cstring>
+#include <mutex>
+#include <vector>
+
+#include "mcpi_internal.h"
+
+namespace mcpi {
+
+int64_t binom(int64_t n, int64_t r)
+{
+    if (r < 0 || n < 0 || r > n) return 0;
+    if (r > n - r) r = n - r;
+    unsigned __int128 acc = 1;
+    for (int64_t t = 1; t <= r; t++) {
+        acc = acc * (unsigned __int128)(n - r + t) / (unsigned __int128)t;
+        if (acc > (unsigned __int128)INT64_MAX) return -1;
+    }
+    return (int64_t)acc;
+}
+
+int64_t nc_of(int N, int M)
+{
+    if (M == 0) return 1;  // reference NC(N, 0) == 1
+    return binom((int64_t)N + M - 1, N);
+}
+
+// ------------------------------------------------------------- kernels --
+
+// target rank from prefix counts of the source row and a move of particles:
+// removed from orbitals a, b (b = -1: none) and added to c, d (d = -1: none).
+__device__ inline int64_t rank_moved(const int *P, const int32_t *B, int N, int M, int64_t nc,
+                                     int a, int b, int c, int d)
+{
+    int64_t s = 0;
+    for (int j = 1; j < M; j++) {
+        int p = P[j] - (a < j) - (b >= 0 && b < j) + (c < j) + (d >= 0 && d < j);
+        s += B[j * (N + 1) + p];
+    }
+    return nc - 1 - s;
+}
+
+// IF table: one thread per configuration, greedy unranking
+// (IndexToFock, configurationsMap.h:238-269).  NCmat columns m >= 1 hold
+// NC(n, m); NC(n, 1) = 1, so the greedy never goes below orbital 1 while k > 0.
+__global__ void k_unrank_if(int N, int M, int64_t nc, const int32_t *__restrict__ NCmat,
+                            const int64_t *__restrict__ idx_list, int32_t *__restrict__ IF)
+{
+    int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= nc) return;
+    int64_t k = idx_list ? idx_list[t] : t;
+    int32_t *v = IF + t * M;
+    for (int j = 0; j < M; j++) v[j] = 0;
+    int n = N, m = M - 1, cnt = 0;
+    while (k > 0) {
+        int64_t cost = NCmat[n + (N + 1) * m];
+        while (k < cost) {
+            if (cnt) { v[m] = cnt; cnt = 0; }
+            m--;
+            cost = NCmat[n + (N + 1) * m];
+        }
+        k -= cost;
+        cnt++;
+        n--;
+    }
+    if (cnt) v[m] += cnt;
+    if (n > 0) v[0] += n;
+}
+
+// ranks of arbitrary states (FockToIndex loop, configurationsMap.h:273-305)
+__global__ void k_rank_states(int N, int M, int64_t n_states, const int32_t *__restrict__ NCmat,
+                              const int32_t *__restrict__ v, int64_t *__restrict__ out)
+{
+    int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= n_states) return;
+    const int32_t *w = v + t * M;
+    int64_t k = 0;
+    int n_left = N;
+    for (int i = M - 1; i > 0; i--)
+        for (int p = w[i]; p > 0; p--) { k += NCmat[n_left + (N + 1) * i]; n_left--; }
+    out[t] = k;
+}
+
+// Single-jump map (OneOneMap, configurationsMap.h:348-417): one thread per
+// row, writes coalesced over rows for every (k, l).
+__global__ void k_one_one_map(mcpi::BasisDev b)
+{
+    extern __shared__ int smem[];
+    const int M = b.M, N = b.N;
+    int32_t *Bt = smem;                               // M*(N+1)
+    int *Ploc = smem + M * (N + 1) + threadIdx.x * (M + 1);
+    for (int t = threadIdx.x; t < M * (N + 1); t += blockDim.x) Bt[t] = b.B[t];
+    __syncthreads();
+    int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= b.nc) return;
+    const int32_t *v = b.IF + i * M;
+    int acc = 0;
+    for (int j = 0; j < M; j++) { Ploc[j] = acc; acc += v[j]; }
+    for (int k = 0; k < M; k++) {
+        bool occ = v[k] >= 1;
+        for (int l = 0; l < M; l++) {
+            int64_t at = i + (int64_t)(k + l * M) * b.nc;
+            int32_t val = -1;
+            if (occ) val = (l == k) ? (int32_t)i : (int32_t)rank_moved(Ploc, Bt, N, M, b.nc, k, -1, l, -1);
+            b.Map[at] = val;
+        }
+    }
+}
+
+// per-row chunk counts: OT = #{k: v_k >= 2}, TT = #{k<s: v_k, v_s >= 1}
+// (allocOneTwoMap 617-650, allocTwoTwoMap 421-460)
+__global__ void k_chunk_counts(int M, int64_t nc, const int32_t *__restrict__ IF,
+                               int32_t *__restrict__ cnt_ot, int32_t *__restrict__ cnt_tt)
+{
+    int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= nc) return;
+    int n1 = 0, n2 = 0;
+    for (int j = 0; j < M; j++) {
+        int x = IF[i * M + j];
+        n1 += (x >= 1);
+        n2 += (x >= 2);
+    }
+    cnt_ot[i] = n2;
+    cnt_tt[i] = n1 * (n1 - 1) / 2;
+}
+
+// ---- exclusive scan of int32 counts into int64 strides (times a factor) --
+#define SCAN_BLOCK 256
+#define SCAN_ITEMS 8
+#define SCAN_TILE (SCAN_BLOCK * SCAN_ITEMS)
+
+__device__ inline int64_t block_exclusive_scan(int64_t x, int64_t *total)
+{
+    __shared__ int64_t wsum[SCAN_BLOCK / 64];
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    int64_t inc = x;
+    for (int o = 1; o < 64; o <<= 1) {
+        int64_t y = __shfl_up(inc, o, 64);
+        if (lane >= o) inc += y;
+    }
+    if (lane == 63) wsum[w] = inc;
+    __syncthreads();
+    int64_t before = 0, tot = 0;
+    for (int t = 0; t < SCAN_BLOCK / 64; t++) {
+        if (t < w) before += wsum[t];
+        tot += wsum[t];
+    }
+    __syncthreads();
+    *total = tot;
+    return before + inc - x;
+}
+
+__global__ void k_scan_reduce(int64_t n, const int32_t *__restrict__ cnt, int64_t *__restrict__ partial)
+{
+    int64_t base = (int64_t)blockIdx.x * SCAN_TILE;
+    int64_t s = 0;
+    for (int t = 0; t < SCAN_ITEMS; t++) {
+        int64_t i = base + (int64_t)t * SCAN_BLOCK + threadIdx.x;
+        if (i < n) s += cnt[i];
+    }
+    int64_t tot;
+    block_exclusive_scan(s, &tot);
+    if (threadIdx.x == 0) partial[blockIdx.x] = tot;
+}
+
+__global__ void k_scan_partials(int64_t nb, int64_t *__restrict__ partial, int64_t *__restrict__ grand)
+{
+    // single block: sequential chunks of SCAN_BLOCK
+    int64_t carry = 0;
+    for (int64_t base = 0; base < nb; base += SCAN_BLOCK) {
+        int64_t i = base + threadIdx.x;
+        int64_t x = (i < nb) ? partial[i] : 0;
+        int64_t tot;
+        int64_t ex = block_exclusive_scan(x, &tot);
+        if (i < nb) partial[i] = carry + ex;
+        carry += tot;
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) *grand = carry;
+}
+
+__global__ void k_scan_final(int64_t n, const int32_t *__restrict__ cnt,
+                             const int64_t *__restrict__ partial, int64_t factor,
+                             int64_t *__restrict__ out)
+{
+    // each thread owns SCAN_ITEMS consecutive entries
+    int64_t base = (int64_t)blockIdx.x * SCAN_TILE + (int64_t)threadIdx.x * SCAN_ITEMS;
+    int32_t loc[SCAN_ITEMS];
+    int64_t s = 0;
+    for (int t = 0; t < SCAN_ITEMS; t++) {
+        int64_t i = base + t;
+        loc[t] = (i < n) ? cnt[i] : 0;
+        s += loc[t];
+    }
+    int64_t tot;
+    int64_t ex = block_exclusive_scan(s, &tot) + partial[blockIdx.x];
+    for (int t = 0; t < SCAN_ITEMS; t++) {
+        int64_t i = base + t;
+        if (i < n) out[i] = ex * factor;
+        ex += loc[t];
+    }
+}
+
+__global__ void k_scan_last(int64_t n, const int64_t *__restrict__ grand, int64_t factor,
+                            int64_t *__restrict__ out)
+{
+    out[n] = (*grand) * factor;
+}
+
+// Double-jump chunk maps, one wavefront per row (lanes over the row's
+// contiguous section, so stores are coalesced).
+//   TT (TwoTwoMap 478-613): chunk c = c-th occupied pair (k<s) in
+//       lexicographic order; entry l + q*M = rank(v - e_k - e_s + e_l + e_q)
+//   OT (OneTwoMap 668-762): chunk c = c-th orbital k with v_k >= 2;
+//       entry l + q*M = rank(v - 2 e_k + e_l + e_q)
+template <bool TT>
+__global__ void k_double_map(mcpi::BasisDev b)
+{
+    extern __shared__ int smem[];
+    const int M = b.M, N = b.N;
+    const int waves = blockDim.x >> 6, w = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    int32_t *Bt = smem;                               // M*(N+1)
+    int *Pw = smem + M * (N + 1) + w * (MCPI_MAXM + 1 + MCPI_MAXM * (MCPI_MAXM - 1) / 2);
+    int *pairs = Pw + MCPI_MAXM + 1;                  // packed k | s << 8
+    for (int t = threadIdx.x; t < M * (N + 1); t += blockDim.x) Bt[t] = b.B[t];
+    __syncthreads();
+    int64_t i = (int64_t)blockIdx.x * waves + w;
+    const bool valid = i < b.nc;
+    const int32_t *v = b.IF + (valid ? i : 0) * M;
+    if (valid && lane == 0) {
+        int acc = 0, np = 0;
+        for (int j = 0; j < M; j++) { Pw[j] = acc; acc += v[j]; }
+        if (TT) {
+            for (int k = 0; k < M; k++) {
+                if (v[k] < 1) continue;
+                for (int s = k + 1; s < M; s++)
+                    if (v[s] >= 1) pairs[np++] = k | (s << 8);
+            }
+        } else {
+            for (int k = 0; k < M; k++)
+                if (v[k] >= 2) pairs[np++] = k | (k << 8);
+        }
+    }
+    __syncthreads();  // every wave of the block reaches this barrier
+    if (!valid) return;
+    const int64_t *stride = TT ? b.strideTT : b.strideOT;
+    int32_t *map = TT ? b.MapTT : b.MapOT;
+    int64_t beg = stride[i], end = stride[i + 1];
+    const int MM = M * M;
+    for (int64_t p = beg + lane; p < end; p += 64) {
+        int64_t off = p - beg;
+        int c = (int)(off / MM), e = (int)(off % MM);
+        int pk = pairs[c];
+        int k = pk & 0xff, s = pk >> 8;
+        int l = e % M, q = e / M;
+        map[p] = (int32_t)rank_moved(Pw, Bt, N, M, b.nc, k, s, l, q);
+    }
+}
+
+}  // namespace mcpi
+
+using namespace mcpi;
+
+// ------------------------------------------------------------ host side --
+
+static int64_t round_up(int64_t x, int64_t a) { return (x + a - 1) / a * a; }
+
+static mcpi_status scan_to_strides(int64_t n, const int32_t *cnt, int64_t factor, int64_t *out,
+                                   int64_t *total_host)
+{
+    int64_t nb = (n + SCAN_TILE - 1) / SCAN_TILE;
+    int64_t *partial = nullptr, *grand = nullptr;
+    MCPI_HIP(hipMalloc(&partial, sizeof(int64_t) * (nb + 1)));
+    MCPI_HIP(hipMalloc(&grand, sizeof(int64_t)));
+    hipLaunchKernelGGL(k_scan_reduce, dim3(nb), dim3(SCAN_BLOCK), 0, 0, n, cnt, partial);
+    hipLaunchKernelGGL(k_scan_partials, dim3(1), dim3(SCAN_BLOCK), 0, 0, nb, partial, grand);
+    hipLaunchKernelGGL(k_scan_final, dim3(nb), dim3(SCAN_BLOCK), 0, 0, n, cnt, partial, factor, out);
+    hipLaunchKernelGGL(k_scan_last, dim3(1), dim3(1), 0, 0, n, grand, factor, out);
+    MCPI_HIP(hipGetLastError());
+    MCPI_HIP(hipMemcpy(total_host, grand, sizeof(int64_t), hipMemcpyDeviceToHost));
+    MCPI_HIP(hipFree(partial));
+    MCPI_HIP(hipFree(grand));
+    return MCPI_OK;
+}
+
+static void host_tables(int N, int M, std::vector<int32_t> &NCmat, std::vector<int32_t> &B)
+{
+    NCmat.assign((size_t)(N + 1) * (M + 1), 0);
+    for (int n = 0; n <= N; n++) {
+        NCmat[n + (N + 1) * 0] = 0;
+        if (M >= 1) NCmat[n + (N + 1) * 1] = 1;
+        for (int m = 2; m <= M; m++) NCmat[n + (N + 1) * m] = (int32_t)nc_of(n, m);
+    }
+    B.assign((size_t)M * (N + 1), 0);
+    for (int j = 1; j < M; j++)
+        for (int p = 0; p <= N; p++) B[j * (N + 1) + p] = (int32_t)binom((int64_t)p + j - 1, j);
+}
+
+static void free_basis(mcpi_basis b)
+{
+    if (!b) return;
+    void *ptrs[] = {b->d.IF, b->d.Map, b->d.MapOT, b->d.MapTT, b->d.strideOT, b->d.strideTT,
+                    b->d.B, b->d.NCmat};
+    for (void *p : ptrs)
+        if (p) (void)hipFree(p);
+    delete b;
+}
+
+static mcpi_status alloc_dev(void **p, int64_t bytes, int64_t *acct)
+{
+    hipError_t e = hipMalloc(p, (size_t)std::max<int64_t>(bytes, 16));
+    if (e != hipSuccess) {
+        (void)hipGetLastError();
+        char msg[160];
+        snprintf(msg, sizeof msg, "device allocation of %lld bytes failed: %s", (long long)bytes,
+                 hipGetErrorString(e));
+        set_error(msg);
+        *p = nullptr;
+        return MCPI_ERR_ALLOC;
+    }
+    *acct += bytes;
+    return MCPI_OK;
+}
+
+// common part of basis creation: tables, IF (computed or uploaded)
+static mcpi_status basis_init(int N, int M, const int32_t *IF_host, const int32_t *NCmat_host,
+                              mcpi_basis *out)
+{
+    MCPI_ARG(N >= 0 && N <= MCPI_MAXN, "N out of range");
+    MCPI_ARG(M >= 1 && M <= MCPI_MAXM, "M out of range (1..32)");
+    int64_t nc = nc_of(N, M);
+    MCPI_ARG(nc > 0 && nc <= INT32_MAX, "number of configurations exceeds int32 indices");
+    mcpi_basis b = new mcpi_basis_s();
+    memset(&b->d, 0, sizeof b->d);
+    b->d.N = N;
+    b->d.M = M;
+    b->d.nc = nc;
+    std::vector<int32_t> NCmat, B;
+    host_tables(N, M, NCmat, B);
+    if (NCmat_host) memcpy(NCmat.data(), NCmat_host, NCmat.size() * sizeof(int32_t));
+    mcpi_status st;
+    if ((st = alloc_dev((void **)&b->d.NCmat, NCmat.size() * 4, &b->device_bytes)) ||
+        (st = alloc_dev((void **)&b->d.B, B.size() * 4 + 4, &b->device_bytes)) ||
+        (st = alloc_dev((void **)&b->d.IF, nc * M * 4, &b->device_bytes))) {
+        free_basis(b);
+        return st;
+    }
+    hipError_t e = hipMemcpy(b->d.NCmat, NCmat.data(), NCmat.size() * 4, hipMemcpyHostToDevice);
+    if (e == hipSuccess) e = hipMemcpy(b->d.B, B.data(), B.size() * 4, hipMemcpyHostToDevice);
+    if (e == hipSuccess && IF_host)
+        e = hipMemcpy(b->d.IF, IF_host, (size_t)nc * M * 4, hipMemcpyHostToDevice);
+    if (e != hipSuccess) {
+        free_basis(b);
+        return hip_fail(e, "basis table upload", __FILE__, __LINE__);
+    }
+    if (!IF_host) {
+        hipLaunchKernelGGL(k_unrank_if, dim3((nc + 255) / 256), dim3(256), 0, 0, N, M, nc,
+                           b->d.NCmat, (const int64_t *)nullptr, b->d.IF);
+        e = hipGetLastError();
+        if (e != hipSuccess) {
+            free_basis(b);
+            return hip_fail(e, "unrank kernel", __FILE__, __LINE__);
+        }
+    }
+    *out = b;
+    return MCPI_OK;
+}
+
+static mcpi_status build_maps_parts(mcpi_basis b, bool want_map, bool want_ot, bool want_tt)
+{
+    const int M = b->d.M, N = b->d.N;
+    const int64_t nc = b->d.nc;
+    if (want_map && !b->d.Map) {
+        MCPI_CHECK(alloc_dev((void **)&b->d.Map, nc * M * M * 4, &b->device_bytes));
+        b->map_len = nc * M * M;
+        size_t sm = sizeof(int) * (M * (N + 1) + 256 * (M + 1));
+        hipLaunchKernelGGL(k_one_one_map, dim3((nc + 255) / 256), dim3(256), sm, 0, b->d);
+        MCPI_HIP(hipGetLastError());
+    }
+    if (want_ot || want_tt) {
+        int32_t *cnt_ot = nullptr, *cnt_tt = nullptr;
+        MCPI_HIP(hipMalloc(&cnt_ot, nc * 4));
+        MCPI_HIP(hipMalloc(&cnt_tt, nc * 4));
+        hipLaunchKernelGGL(k_chunk_counts, dim3((nc + 255) / 256), dim3(256), 0, 0, M, nc, b->d.IF,
+                           cnt_ot, cnt_tt);
+        MCPI_HIP(hipGetLastError());
+        const int waves = 4;
+        size_t sm = sizeof(int) * (M * (N + 1) +
+                                   waves * (MCPI_MAXM + 1 + MCPI_MAXM * (MCPI_MAXM - 1) / 2));
+        int64_t grid = (nc + waves - 1) / waves;
+        if (want_ot && !b->d.MapOT) {
+            MCPI_CHECK(alloc_dev((void **)&b->d.strideOT, (nc + 1) * 8, &b->device_bytes));
+            MCPI_CHECK(scan_to_strides(nc, cnt_ot, (int64_t)M * M, b->d.strideOT, &b->ot_len));
+            MCPI_CHECK(alloc_dev((void **)&b->d.MapOT, b->ot_len * 4, &b->device_bytes));
+            if (b->ot_len > 0)
+                hipLaunchKernelGGL(k_double_map<false>, dim3(grid), dim3(64 * waves), sm, 0, b->d);
+            MCPI_HIP(hipGetLastError());
+        }
+        if (want_tt && !b->d.MapTT) {
+            MCPI_CHECK(alloc_dev((void **)&b->d.strideTT, (nc + 1) * 8, &b->device_bytes));
+            MCPI_CHECK(scan_to_strides(nc, cnt_tt, (int64_t)M * M, b->d.strideTT, &b->tt_len));
+            MCPI_CHECK(alloc_dev((void **)&b->d.MapTT, b->tt_len * 4, &b->device_bytes));
+            if (b->tt_len > 0)
+                hipLaunchKernelGGL(k_double_map<true>, dim3(grid), dim3(64 * waves), sm, 0, b->d);
+            MCPI_HIP(hipGetLastError());
+        }
+        MCPI_HIP(hipFree(cnt_ot));
+        MCPI_HIP(hipFree(cnt_tt));
+    }
+    if (b->d.Map && b->d.MapOT && b->d.MapTT) b->maps = MCPI_MAPS_FULL;
+    else if (b->d.Map) b->maps = MCPI_MAPS_SINGLE;
+    else b->maps = MCPI_MAPS_NONE;
+    MCPI_HIP(hipDeviceSynchronize());
+    return MCPI_OK;
+}
+
+static mcpi_status basis_build_maps(mcpi_basis b, int maps)
+{
+    return build_maps_parts(b, maps >= MCPI_MAPS_SINGLE, maps >= MCPI_MAPS_FULL, maps >= MCPI_MAPS_FULL);
+}
+
+namespace mcpi {
+
+// Upload of caller-provided tables (reference-shaped entry points).  int32
+// strides of the reference become int64 with a closing entry.
+mcpi_status basis_upload_raw(int N, int M, const int32_t *IF, const int32_t *NCmat,
+                             const int32_t *Map, const int32_t *MapOT, const int32_t *MapTT,
+                             const int *strideOT, const int *strideTT, int64_t ot_len,
+                             int64_t tt_len, mcpi_basis *out)
+{
+    mcpi_basis b = nullptr;
+    MCPI_CHECK(basis_init(N, M, IF, NCmat, &b));
+    const int64_t nc = b->d.nc;
+    mcpi_status st = MCPI_OK;
+    b->maps = MCPI_MAPS_NONE;
+    if (Map) {
+        b->maps = MCPI_MAPS_SINGLE;
+        b->map_len = nc * M * M;
+        st = alloc_dev((void **)&b->d.Map, b->map_len * 4, &b->device_bytes);
+        if (!st && hipMemcpy(b->d.Map, Map, b->map_len * 4, hipMemcpyHostToDevice) != hipSuccess)
+            st = MCPI_ERR_HIP;
+    }
+    if (!st && MapOT && MapTT && strideOT && strideTT) {
+        b->maps = MCPI_MAPS_FULL;
+        b->ot_len = ot_len;
+        b->tt_len = tt_len;
+        std::vector<int64_t> s64(nc + 1);
+        st = alloc_dev((void **)&b->d.strideOT, (nc + 1) * 8, &b->device_bytes);
+        if (!st) st = alloc_dev((void **)&b->d.strideTT, (nc + 1) * 8, &b->device_bytes);
+        if (!st) st = alloc_dev((void **)&b->d.MapOT, ot_len * 4, &b->device_bytes);
+        if (!st) st = alloc_dev((void **)&b->d.MapTT, tt_len * 4, &b->device_bytes);
+        if (!st) {
+            for (int64_t i = 0; i < nc; i++) s64[i] = strideOT[i];
+            s64[nc] = ot_len;
+            if (hipMemcpy(b->d.strideOT, s64.data(), (nc + 1) * 8, hipMemcpyHostToDevice) != hipSuccess)
+                st = MCPI_ERR_HIP;
+            for (int64_t i = 0; i < nc; i++) s64[i] = strideTT[i];
+            s64[nc] = tt_len;
+            if (!st && hipMemcpy(b->d.strideTT, s64.data(), (nc + 1) * 8, hipMemcpyHostToDevice) != hipSuccess)
+                st = MCPI_ERR_HIP;
+            if (!st && ot_len && hipMemcpy(b->d.MapOT, MapOT, ot_len * 4, hipMemcpyHostToDevice) != hipSuccess)
+                st = MCPI_ERR_HIP;
+            if (!st && tt_len && hipMemcpy(b->d.MapTT, MapTT, tt_len * 4, hipMemcpyHostToDevice) != hipSuccess)
+                st = MCPI_ERR_HIP;
+        }
+    }
+    if (st) {
+        if (st == MCPI_ERR_HIP) set_error("upload of reference tables failed");
+        free_basis(b);
+        return st;
+    }
+    *out = b;
+    return MCPI_OK;
+}
+
+mcpi_status basis_build_parts(mcpi_basis b, int want_map, int want_ot, int want_tt)
+{
+    MCPI_ARG(b, "null basis");
+    return build_maps_parts(b, want_map != 0, want_ot != 0, want_tt != 0);
+}
+
+}  // namespace mcpi
+
+extern "C" {
+
+mcpi_status mcpi_hip_basis_create(int N, int M, int maps, mcpi_basis *out)
+{
+    MCPI_ARG(out, "null output handle");
+    MCPI_ARG(maps >= MCPI_MAPS_NONE && maps <= MCPI_MAPS_FULL, "maps must be 0, 1 or 2");
+    mcpi_basis b = nullptr;
+    MCPI_CHECK(basis_init(N, M, nullptr, nullptr, &b));
+    mcpi_status st = basis_build_maps(b, maps);
+    if (st != MCPI_OK) {
+        free_basis(b);
+        return st;
+    }
+    *out = b;
+    return MCPI_OK;
+}
+
+mcpi_status mcpi_hip_basis_destroy(mcpi_basis b)
+{
+    if (b) {
+        (void)hipDeviceSynchronize();
+        free_basis(b);
+    }
+    return MCPI_OK;
+}
+
+mcpi_status mcpi_hip_basis_info(mcpi_basis b, mcpi_basis_info *info)
+{
+    MCPI_ARG(b && info, "null argument");
+    info->N = b->d.N;
+    info->M = b->d.M;
+    info->maps = b->maps;
+    info->nc = b->d.nc;
+    info->map_len = b->map_len;
+    info->ot_len = b->ot_len;
+    info->tt_len = b->tt_len;
+    info->device_bytes = b->device_bytes;
+    return MCPI_OK;
+}
+
+mcpi_status mcpi_hip_basis_copy_ncmat(mcpi_basis b, int32_t *NCmat)
+{
+    MCPI_ARG(b && NCmat, "null argument");
+    MCPI_HIP(hipMemcpy(NCmat, b->d.NCmat, sizeof(int32_t) * (b->d.N + 1) * (b->d.M + 1),
+                       hipMemcpyDeviceToHost));
+    return MCPI_OK;
+}
+
+mcpi_status mcpi_hip_basis_copy_if(mcpi_basis b, int32_t *IF)
+{
+    MCPI_ARG(b && IF, "null argument");
+    MCPI_HIP(hipMemcpy(IF, b->d.IF, sizeof(int32_t) * b->d.nc * b->d.M, hipMemcpyDeviceToHost));
+    return MCPI_OK;
+}
+
+mcpi_status mcpi_hip_basis_copy_map(mcpi_basis b, int32_t *Map)
+{
+    MCPI_ARG(b && Map, "null argument");
+    MCPI_ARG(b->d.Map, "basis has no single-jump map");
+    MCPI_HIP(hipMemcpy(Map, b->d.Map, sizeof(int32_t) * b->map_len, hipMemcpyDeviceToHost));
+    return MCPI_OK;
+}
+
+mcpi_status mcpi_hip_basis_copy_ot(mcpi_basis b, int64_t *strideOT, int32_t *MapOT)
+{
+    MCPI_ARG(b, "null basis");
+    MCPI_ARG(b->d.MapOT, "basis has no double-jump maps");
+    if (strideOT)
+        MCPI_HIP(hipMemcpy(strideOT, b->d.strideOT, sizeof(int64_t) * b->d.nc, hipMemcpyDeviceToHost));
+    if (MapOT && b->ot_len)
+        MCPI_HIP(hipMemcpy(MapOT, b->d.MapOT, sizeof(int32_t) * b->ot_len, hipMemcpyDeviceToHost));
+    return MCPI_OK;
+}
+
+mcpi_status mcpi_hip_basis_copy_tt(mcpi_basis b, int64_t *strideTT, int32_t *MapTT)
+{
+    MCPI_ARG(b, "null basis");
+    MCPI_ARG(b->d.MapTT, "basis has no double-jump maps");
+    if (strideTT)
+        MCPI_HIP(hipMemcpy(strideTT, b->d.strideTT, sizeof(int64_t) * b->d.nc, hipMemcpyDeviceToHost));
+    if (MapTT && b->tt_len)
+        MCPI_HIP(hipMemcpy(MapTT, b->d.MapTT, sizeof(int32_t) * b->tt_len, hipMemcpyDeviceToHost));
+    return MCPI_OK;
+}
+
+mcpi_status mcpi_hip_basis_copy_range(mcpi_basis b, int which, int64_t offset, int64_t count, void *dst)
+{
+    MCPI_ARG(b && dst && offset >= 0 && count >= 0, "bad argument");
+    const void *src = nullptr;
+    int64_t len = 0, esize = 4;
+    switch (which) {
+    case 0: src = b->d.Map; len = b->map_len; break;
+    case 1: src = b->d.MapOT; len = b->ot_len; break;
+    case 2: src = b->d.MapTT; len = b->tt_len; break;
+    case 3: src = b->d.IF; len = b->d.nc * b->d.M; break;
+    case 4: src = b->d.strideOT; len = b->d.strideOT ? b->d.nc + 1 : 0; esize = 8; break;
+    case 5: src = b->d.strideTT; len = b->d.strideTT ? b->d.nc + 1 : 0; esize = 8; break;
+    default: MCPI_ARG(false, "which must be 0..5");
+    }
+    MCPI_ARG(src, "table not stored in this basis");
+    MCPI_ARG(offset + count <= len, "range beyond the table");
+    if (count)
+        MCPI_HIP(hipMemcpy(dst, (const char *)src + offset * esize, count * esize, hipMemcpyDeviceToHost));
+    return MCPI_OK;
+}
+
+mcpi_status mcpi_hip_basis_rank(mcpi_basis b, int64_t n, const int32_t *v, int64_t *idx)
+{
+    MCPI_ARG(b && v && idx && n >= 0, "bad argument");
+    if (n == 0) return MCPI_OK;
+    const int M = b->d.M, N = b->d.N;
+    for (int64_t t = 0; t < n; t++) {
+        int64_t s = 0;
+        for (int j = 0; j < M; j++) {
+            MCPI_ARG(v[t * M + j] >= 0, "negative occupation");
+            s += v[t * M + j];
+        }
+        MCPI_ARG(s == N, "occupation vector does not hold N particles");
+    }
+    int32_t *dv = nullptr;
+    int64_t *di = nullptr;
+    MCPI_HIP(hipMalloc(&dv, n * M * 4));
+    MCPI_HIP(hipMalloc(&di, n * 8));
+    MCPI_HIP(hipMemcpy(dv, v, n * M * 4, hipMemcpyHostToDevice));
+    hipLaunchKernelGGL(k_rank_states, dim3((n + 255) / 256), dim3(256), 0, 0, N, M, n, b->d.NCmat, dv, di);
+    MCPI_HIP(hipGetLastError());
+    MCPI_HIP(hipMemcpy(idx, di, n * 8, hipMemcpyDeviceToHost));
+    MCPI_HIP(hipFree(dv));
+    MCPI_HIP(hipFree(di));
+    return MCPI_OK;
+}
+
+mcpi_status mcpi_hip_basis_unrank(mcpi_basis b, int64_t n, const int64_t *idx, int32_t *v)
+{
+    MCPI_ARG(b && v && idx && n >= 0, "bad argument");
+    if (n == 0) return MCPI_OK;
+    for (int64_t t = 0; t < n; t++) MCPI_ARG(idx[t] >= 0 && idx[t] < b->d.nc, "index out of range");
+    const int M = b->d.M, N = b->d.N;
+    int32_t *dv = nullptr;
+    int64_t *di = nullptr;
+    MCPI_HIP(hipMalloc(&dv, n * M * 4));
+    MCPI_HIP(hipMalloc(&di, n * 8));
+    MCPI_HIP(hipMemcpy(di, idx, n * 8, hipMemcpyHostToDevice));
+    hipLaunchKernelGGL(k_unrank_if, dim3((n + 255) / 256), dim3(256), 0, 0, N, M, n, b->d.NCmat,
+                       (const int64_t *)di, dv);
+    MCPI_HIP(hipGetLastError());
+    MCPI_HIP(hipMemcpy(v, dv, n * M * 4, hipMemcpyDeviceToHost));
+    MCPI_HIP(hipFree(dv));
+    MCPI_HIP(hipFree(di));
+    return MCPI_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,160 @@
+// Internal definitions shared by the HIP sources of libmcpi_hip.so.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include <string>
+
+#include "../../include/mcpi_hip.h"
+
+// ---------------------------------------------------------------- limits --
+// Orbitals and particles a basis may have.  The per-row occupation of a
+// state is held in 8-bit lanes, and tables indexed by orbital are sized for
+// MCPI_MAXM.
+#define MCPI_MAXM 32
+#define MCPI_MAXN 120
+
+// ----------------------------------------------------------- error state --
+namespace mcpi {
+
+void set_error(const std::string &msg);
+mcpi_status hip_fail(hipError_t e, const char *what, const char *file, int line);
+
+}  // namespace mcpi
+
+#define MCPI_HIP(call)                                                        \
+    do {                                                                      \
+        hipError_t _e = (call);                                               \
+        if (_e != hipSuccess) return mcpi::hip_fail(_e, #call, __FILE__, __LINE__); \
+    } while (0)
+
+#define MCPI_CHECK(call)                                                      \
+    do {                                                                      \
+        mcpi_status _s = (call);                                              \
+        if (_s != MCPI_OK) return _s;                                         \
+    } while (0)
+
+#define MCPI_ARG(cond, msg)                                                   \
+    do {                                                                      \
+        if (!(cond)) { mcpi::set_error(msg); return MCPI_ERR_ARG; }           \
+    } while (0)
+
+// --------------------------------------------------------------- complex --
+struct cplx {
+    double re, im;
+};
+
+__host__ __device__ inline cplx cmake(double r, double i) { cplx z; z.re = r; z.im = i; return z; }
+__host__ __device__ inline cplx cadd(cplx a, cplx b) { return cmake(a.re + b.re, a.im + b.im); }
+__host__ __device__ inline cplx cscale(cplx a, double s) { return cmake(a.re * s, a.im * s); }
+// acc += a * b
+__host__ __device__ inline cplx cfma(cplx a, cplx b, cplx acc)
+{
+    acc.re = fma(a.re, b.re, acc.re);
+    acc.re = fma(-a.im, b.im, acc.re);
+    acc.im = fma(a.re, b.im, acc.im);
+    acc.im = fma(a.im, b.re, acc.im);
+    return acc;
+}
+// acc += conj(a) * b
+__host__ __device__ inline cplx cfma_conj(cplx a, cplx b, cplx acc)
+{
+    acc.re = fma(a.re, b.re, acc.re);
+    acc.re = fma(a.im, b.im, acc.re);
+    acc.im = fma(a.re, b.im, acc.im);
+    acc.im = fma(-a.im, b.re, acc.im);
+    return acc;
+}
+
+namespace mcpi {
+
+// --------------------------------------------------------- combinatorics --
+// Exact C(n, r) in int64 (-1 on overflow).
+int64_t binom(int64_t n, int64_t r);
+// NC(N, M) = C(N+M-1, N), the reference's NC (configurationsMap.h:145-202)
+// without the INT32 guard; NC(N, 0) = 1 like the reference.
+int64_t nc_of(int N, int M);
+
+// Device-side description of a basis: everything a kernel needs to rank,
+// unrank and walk the maps.  Lives in device memory (basis.hip).
+//   rank(v) = nc - 1 - sum_{j=1}^{M-1} B[j][P_j],  P_j = sum_{o<j} v_o,
+//   B[j][p] = C(p + j - 1, j)
+// (closed form of the FockToIndex loop, configurationsMap.h:273-305).
+struct BasisDev {
+    int N, M;
+    int64_t nc;
+    int32_t *IF;        // nc*M
+    int32_t *Map;       // nc*M*M or null
+    int32_t *MapOT;     // or null
+    int32_t *MapTT;     // or null
+    int64_t *strideOT;  // nc+1 entries (last = |MapOT|) or null
+    int64_t *strideTT;  // nc+1 entries (last = |MapTT|) or null
+    int32_t *B;         // (M) x (N+1) table B[j*(N+1) + p], row 0 unused
+    int32_t *NCmat;     // (N+1)*(M+1), reference layout
+};
+
+}  // namespace mcpi
+
+// internal basis/hamiltonian handles
+struct mcpi_basis_s {
+    mcpi::BasisDev d;
+    int maps;
+    int64_t ot_len, tt_len, map_len;
+    int64_t device_bytes;
+};
+
+// Folded coefficient tables of a Hamiltonian (device memory), produced on
+// the host from Ho and Hint by the rule folding of hamiltonianMatrix.h:
+// 1494-1963.  For a row with occupations v (r1[q] = sqrt(v_q+1),
+// r2[q] = sqrt((v_q+1)(v_q+2))):
+//   diag      = sum_k dk[k] v_k + sum_k d1[k] v_k(v_k-1) + sum_{k<s} d2[k,s] v_k v_s
+//   1-jump k->l (v_k>=1, l!=k): sqrt(v_k) r1[l] (A0[k,l] + sum_s v_s A[k,l,s])
+//   OT (k; q<=l, q,l != k, v_k>=2): sqrt(v_k(v_k-1)) * ot[k][q+l*M] * (q==l ? r2[q] : r1[q] r1[l])
+//   TT (k<s; q<=l not in {k,s}):  sqrt(v_k v_s) * tt[p][q+l*M] * (same)
+struct HamCoefDev {
+    cplx *dk;    // M
+    cplx *d1;    // M
+    cplx *d2;    // M*M  (k + s*M, k < s)
+    cplx *A0;    // M*M  (k + l*M)
+    cplx *A;     // M*M*M  ((k + l*M)*M + s)
+    cplx *ot;    // M * M*M
+    cplx *tt;    // P * M*M, P = M(M-1)/2 pairs in lexicographic order
+    uint8_t *nz_1j;  // M*M: 1 if any coefficient of the jump k->l is non-zero
+    uint8_t *nz_ot;  // M*M*M: per (k, e)
+    uint8_t *nz_tt;  // P*M*M: per (p, e)
+};
+
+struct mcpi_ham_s {
+    mcpi_basis basis;
+    HamCoefDev c;
+    void *coef_block;   // single device allocation holding the tables
+    cplx *Ho_dev;       // M*M flat, k + M*l
+    cplx *Hint_dev;     // M^4
+    int skip_zero;
+    int schedule;
+    int tuned_schedule;
+};
+
+namespace mcpi {
+
+__host__ __device__ inline int pair_index(int k, int s, int M)
+{
+    // lexicographic index of k<s among M(M-1)/2 pairs
+    return k * (2 * M - k - 1) / 2 + (s - k - 1);
+}
+
+// kernels launched from host code of other translation units
+mcpi_status launch_apply(mcpi_ham h, const cplx *C, cplx *out, hipStream_t st);
+mcpi_status launch_apply_raw(const BasisDev &b, int maps, const HamCoefDev &c, int skip_zero,
+                             int schedule, const cplx *C, cplx *out, hipStream_t st);
+mcpi_status build_coefficients(int M, const cplx *Ho, const cplx *Hint, HamCoefDev *c,
+                               void **block);
+mcpi_status basis_upload_raw(int N, int M, const int32_t *IF, const int32_t *NCmat,
+                             const int32_t *Map, const int32_t *MapOT, const int32_t *MapTT,
+                             const int *strideOT, const int *strideTT, int64_t ot_len,
+                             int64_t tt_len, mcpi_basis *out);
+// build the requested maps of a basis whose IF table is in place
+mcpi_status basis_build_parts(mcpi_basis b, int want_map, int want_ot, int want_tt);
+
+}  // namespace mcpi

@@ -1,0 +1,356 @@
+// One- and two-body density matrices and the Hellmann-Feynman adjoint.
+//
+// Reference: OBrho_XM, onebodyMatrix.h:190-251 (rho1), and TBrho_XX,
+// twobodyMatrix.h:991-1487 (rho2, rules 1-8 plus symmetry copies).
+//
+// The reference loops orbital-outer / row-inner.  Here a block walks row
+// tiles (lanes = rows) and, for every representative term of a class, sums
+// the lanes' contributions with a wavefront reduction and adds the result to
+// an LDS accumulator.  Each block writes its accumulators once; a second
+// kernel sums the blocks in a fixed order, so the result is deterministic.
+// The host expands the representative terms into full matrices with the
+// reference's symmetry copies.
+//
+// Term classes (compact layouts, P = M(M-1)/2 pairs, pq = pair index):
+//   class 0 (single jumps and diagonal):
+//     r1d[k] | r1o[kl] | R1[k] | R2[ks] | R4[kl] | R5[ks] | R7[kl][s]
+//   class 1 (same-orbital double jumps):  R3[kq] | R6[k][ql]
+//   class 2 (two-orbital double jumps):   R8[ks - p0][ql]   (pairs p0..p1-1)
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <cstring>
+#include <vector>
+
+#include "mcpi_internal.h"
+
+namespace mcpi {
+
+__device__ inline cplx ldcr(const cplx *p)
+{
+    double2 t = *reinterpret_cast<const double2 *>(p);
+    return cmake(t.x, t.y);
+}
+
+__device__ inline double wave_sum(double x)
+{
+    for (int o = 32; o > 0; o >>= 1) x += __shfl_xor(x, o, 64);
+    return x;
+}
+
+// add the wave total of z to acc[idx] (one LDS atomic pair per wave)
+__device__ inline void wave_add(cplx z, cplx *acc, int idx, bool any)
+{
+    if (!any) return;  // uniform: no lane contributes
+    double re = wave_sum(z.re), im = wave_sum(z.im);
+    if ((threadIdx.x & 63) == 0) {
+        atomicAdd(&acc[idx].re, re);
+        atomicAdd(&acc[idx].im, im);
+    }
+}
+
+__device__ inline bool wave_any(bool p) { return __ballot(p) != 0; }
+
+template <int CLASS>
+__global__ __launch_bounds__(256) void k_rho(BasisDev b, const cplx *__restrict__ C, int p0, int p1,
+                                            int nterms, cplx *__restrict__ partial)
+{
+    extern __shared__ double smem_r[];
+    const int M = b.M, N = b.N, T = blockDim.x, t = threadIdx.x;
+    const int P = M * (M - 1) / 2;
+    const int64_t nc = b.nc, MM = (int64_t)M * M;
+    cplx *acc = (cplx *)smem_r;                       // nterms
+    double *sq = (double *)(acc + nterms);            // N+3
+    uint8_t *Vs = (uint8_t *)(sq + N + 3);            // M*T
+    for (int x = t; x < nterms; x += T) acc[x] = cmake(0, 0);
+    for (int x = t; x < N + 3; x += T) sq[x] = sqrt((double)x);
+    __syncthreads();
+
+    for (int64_t tile = (int64_t)blockIdx.x * T; tile < nc; tile += (int64_t)gridDim.x * T) {
+        const int64_t i = tile + t;
+        const bool valid = i < nc;
+        for (int j = 0; j < M; j++) Vs[j * T + t] = (uint8_t)(valid ? b.IF[i * M + j] : 0);
+        __syncthreads();
+#define V(j) ((int)Vs[(j) * T + t])
+        const cplx ci = valid ? ldcr(C + i) : cmake(0, 0);
+        const cplx cci = cmake(ci.re, -ci.im);
+        const double m2 = ci.re * ci.re + ci.im * ci.im;
+        if (CLASS == 0) {
+            int o = 0;
+            // r1d, R1
+            for (int k = 0; k < M; k++) {
+                int vk = V(k);
+                wave_add(cmake(m2 * vk, 0), acc, o + k, true);
+                wave_add(cmake(m2 * vk * (vk - 1), 0), acc, o + P + M + k, wave_any(vk >= 2));
+            }
+            o += M;  // r1o at o
+            const int o_r1o = o, o_R2 = M + P + M, o_R4 = o_R2 + P, o_R5 = o_R4 + P, o_R7 = o_R5 + P;
+            int pq = 0;
+            for (int k = 0; k < M; k++)
+                for (int l = k + 1; l < M; l++, pq++) {
+                    const int vk = V(k), vl = V(l);
+                    // R2: |C|^2 v_k v_l over all rows
+                    wave_add(cmake(m2 * vk * vl, 0), acc, o_R2 + pq, wave_any(vk && vl));
+                    const bool act = valid && vk >= 1;
+                    if (!wave_any(act)) continue;
+                    cplx g = cmake(0, 0);
+                    if (act) {
+                        int64_t j = b.Map[i + (int64_t)(k + M * l) * nc];
+                        g = cfma_conj(ci, ldcr(C + j), cmake(0, 0));   // conj(C_i) C_j
+                    }
+                    const double root = act ? sq[vk] * sq[vl + 1] : 0.0;
+                    wave_add(cscale(g, root), acc, o_r1o + pq, true);
+                    wave_add(cscale(g, act && vk >= 2 ? (vk - 1) * root : 0.0), acc, o_R4 + pq,
+                             wave_any(act && vk >= 2));
+                    wave_add(cscale(g, act && vl >= 1 ? vl * root : 0.0), acc, o_R5 + pq,
+                             wave_any(act && vl >= 1));
+                    for (int s = 0; s < M; s++) {
+                        if (s == k || s == l) continue;
+                        const int vs = V(s);
+                        wave_add(cscale(g, act && vs >= 1 ? vs * root : 0.0), acc, o_R7 + pq * M + s,
+                                 wave_any(act && vs >= 1));
+                    }
+                }
+        } else if (CLASS == 1) {
+            // R3 (k < q) and R6 (q < l, k not in {q, l}), v_k >= 2
+            int cot = 0;
+            for (int k = 0; k < M; k++) {
+                const int vk = V(k);
+                const bool act = valid && vk >= 2;
+                const int64_t base = act ? b.strideOT[i] + (int64_t)cot * MM : 0;
+                if (act) cot++;
+                if (!wave_any(act)) continue;
+                const double fk = act ? sq[vk] * sq[vk - 1] : 0.0;
+                for (int q = k + 1; q < M; q++) {
+                    cplx z = cmake(0, 0);
+                    if (act) {
+                        int64_t j = b.MapOT[base + q + q * M];
+                        z = cscale(cfma_conj(ci, ldcr(C + j), cmake(0, 0)),
+                                   fk * sq[V(q) + 1] * sq[V(q) + 2]);
+                    }
+                    wave_add(z, acc, pair_index(k, q, M), true);
+                }
+                int pq = 0;
+                for (int q = 0; q < M; q++)
+                    for (int l = q + 1; l < M; l++, pq++) {
+                        if (q == k || l == k) continue;
+                        cplx z = cmake(0, 0);
+                        if (act) {
+                            int64_t j = b.MapOT[base + l + q * M];
+                            z = cscale(cfma_conj(ci, ldcr(C + j), cmake(0, 0)),
+                                       fk * sq[V(q) + 1] * sq[V(l) + 1]);
+                        }
+                        wave_add(z, acc, P + k * P + pq, true);
+                    }
+            }
+        } else {
+            // R8: k < s occupied, q < l, {q, l} disjoint from {k, s}
+            int ctt = 0, p = 0;
+            for (int k = 0; k < M; k++)
+                for (int s = k + 1; s < M; s++, p++) {
+                    const int vk = V(k), vs = V(s);
+                    const bool act = valid && vk >= 1 && vs >= 1;
+                    const int64_t base = act ? b.strideTT[i] + (int64_t)ctt * MM : 0;
+                    if (act) ctt++;
+                    if (p < p0 || p >= p1) continue;
+                    if (!wave_any(act)) continue;
+                    const double fks = act ? sq[vk] * sq[vs] : 0.0;
+                    int pq = 0;
+                    for (int q = 0; q < M; q++)
+                        for (int l = q + 1; l < M; l++, pq++) {
+                            if (q == k || q == s || l == k || l == s) continue;
+                            cplx z = cmake(0, 0);
+                            if (act) {
+                                int64_t j = b.MapTT[base + q + l * M];
+                                z = cscale(cfma_conj(ci, ldcr(C + j), cmake(0, 0)),
+                                           fks * sq[V(q) + 1] * sq[V(l) + 1]);
+                            }
+                            wave_add(z, acc, (p - p0) * P + pq, true);
+                        }
+                }
+        }
+#undef V
+        __syncthreads();
+    }
+    for (int x = t; x < nterms; x += T) partial[(int64_t)blockIdx.x * nterms + x] = acc[x];
+}
+
+__global__ void k_sum_blocks(int nblocks, int nterms, const cplx *__restrict__ partial, cplx *__restrict__ out)
+{
+    int x = blockIdx.x * blockDim.x + threadIdx.x;
+    if (x >= nterms) return;
+    double re = 0, im = 0;
+    for (int bl = 0; bl < nblocks; bl++) {
+        cplx z = partial[(int64_t)bl * nterms + x];
+        re += z.re;
+        im += z.im;
+    }
+    out[x] = cmake(re, im);
+}
+
+// run one class and bring its compact terms to the host
+static mcpi_status run_class(mcpi_basis bs, int cls, const cplx *C, int p0, int p1, int nterms,
+                             std::vector<cplx> &host)
+{
+    const BasisDev &b = bs->d;
+    const int T = 256;
+    int dev = 0, ncu = 0;
+    MCPI_HIP(hipGetDevice(&dev));
+    MCPI_HIP(hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev));
+    int64_t tiles = (b.nc + T - 1) / T;
+    int nblocks = (int)std::min<int64_t>(tiles, 2 * (int64_t)ncu);
+    size_t sm = sizeof(cplx) * nterms + sizeof(double) * (b.N + 3) + (size_t)b.M * T + 16;
+    size_t max_sm = 160 * 1024;
+    MCPI_ARG(sm <= max_sm, "density-matrix accumulators exceed LDS");
+    cplx *partial = nullptr, *out = nullptr;
+    MCPI_HIP(hipMalloc(&partial, sizeof(cplx) * (size_t)nblocks * nterms));
+    MCPI_HIP(hipMalloc(&out, sizeof(cplx) * nterms));
+    if (cls == 0) {
+        MCPI_HIP(hipFuncSetAttribute((const void *)k_rho<0>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)sm));
+        hipLaunchKernelGGL(k_rho<0>, dim3(nblocks), dim3(T), sm, 0, b, C, p0, p1, nterms, partial);
+    } else if (cls == 1) {
+        MCPI_HIP(hipFuncSetAttribute((const void *)k_rho<1>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)sm));
+        hipLaunchKernelGGL(k_rho<1>, dim3(nblocks), dim3(T), sm, 0, b, C, p0, p1, nterms, partial);
+    } else {
+        MCPI_HIP(hipFuncSetAttribute((const void *)k_rho<2>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)sm));
+        hipLaunchKernelGGL(k_rho<2>, dim3(nblocks), dim3(T), sm, 0, b, C, p0, p1, nterms, partial);
+    }
+    MCPI_HIP(hipGetLastError());
+    hipLaunchKernelGGL(k_sum_blocks, dim3((nterms + 255) / 256), dim3(256), 0, 0, nblocks, nterms, partial, out);
+    MCPI_HIP(hipGetLastError());
+    host.resize(nterms);
+    MCPI_HIP(hipMemcpy(host.data(), out, sizeof(cplx) * nterms, hipMemcpyDeviceToHost));
+    MCPI_HIP(hipFree(partial));
+    MCPI_HIP(hipFree(out));
+    return MCPI_OK;
+}
+
+static inline int64_t I4(int a, int b, int c, int d, int M)
+{
+    return a + (int64_t)M * (b + (int64_t)M * (c + (int64_t)M * d));
+}
+static inline cplx conjc(cplx z) { return cmake(z.re, -z.im); }
+
+// rho1 / rho2 from the device; either output may be null
+static mcpi_status densities(mcpi_basis bs, const cplx *C, cplx *rho1, cplx *rho2)
+{
+    const int M = bs->d.M;
+    const int P = M * (M - 1) / 2;
+    MCPI_ARG(bs->d.Map, "density matrices need the single-jump map");
+    std::vector<cplx> a;
+    const int nA = M + P + M + P + P + P + P * M;
+    MCPI_CHECK(run_class(bs, 0, C, 0, 0, nA, a));
+    const cplx *r1d = a.data(), *r1o = r1d + M, *R1 = r1o + P, *R2 = R1 + M, *R4 = R2 + P,
+               *R5 = R4 + P, *R7 = R5 + P;
+    if (rho1) {
+        for (int k = 0; k < M; k++) {
+            rho1[k + M * k] = r1d[k];
+            for (int l = k + 1; l < M; l++) {
+                cplx r = r1o[pair_index(k, l, M)];
+                rho1[k + M * l] = r;        // <a*_k a_l>
+                rho1[l + M * k] = conjc(r);
+            }
+        }
+    }
+    if (!rho2) return MCPI_OK;
+    MCPI_ARG(bs->d.MapOT && bs->d.MapTT, "two-body density needs the double-jump maps");
+    std::vector<cplx> ob, tt((size_t)P * P);
+    MCPI_CHECK(run_class(bs, 1, C, 0, 0, P + M * P, ob));
+    // class 2 in pair ranges that fit in LDS (about 120 KB of accumulators)
+    int chunk = std::max(1, (int)(120 * 1024 / (16 * std::max(P, 1))));
+    for (int p0 = 0; p0 < P; p0 += chunk) {
+        int p1 = std::min(P, p0 + chunk);
+        std::vector<cplx> part;
+        MCPI_CHECK(run_class(bs, 2, C, p0, p1, (p1 - p0) * P, part));
+        memcpy(tt.data() + (size_t)p0 * P, part.data(), sizeof(cplx) * part.size());
+    }
+    const cplx *R3 = ob.data(), *R6 = R3 + P;
+    const int64_t M4 = (int64_t)M * M * M * M;
+    for (int64_t x = 0; x < M4; x++) rho2[x] = cmake(0, 0);
+    // expansion by the reference's symmetry copies, in its rule order
+    // (twobodyMatrix.h:1050, 1073-1076, 1113-1114, 1141-1144, 1171-1174,
+    //  1214-1217 (x3), 1335-1343 (x3), 1477-1480)
+    for (int k = 0; k < M; k++) rho2[I4(k, k, k, k, M)] = R1[k];
+    for (int k = 0; k < M; k++)
+        for (int s = k + 1; s < M; s++) {
+            cplx r = R2[pair_index(k, s, M)];
+            rho2[I4(k, s, k, s, M)] = r; rho2[I4(s, k, k, s, M)] = r;
+            rho2[I4(s, k, s, k, M)] = r; rho2[I4(k, s, s, k, M)] = r;
+        }
+    for (int k = 0; k < M; k++)
+        for (int q = k + 1; q < M; q++) {
+            cplx r = R3[pair_index(k, q, M)];
+            rho2[I4(k, k, q, q, M)] = r; rho2[I4(q, q, k, k, M)] = conjc(r);
+        }
+    for (int k = 0; k < M; k++)
+        for (int l = k + 1; l < M; l++) {
+            cplx r = R4[pair_index(k, l, M)];
+            rho2[I4(k, k, k, l, M)] = r; rho2[I4(k, k, l, k, M)] = r;
+            rho2[I4(l, k, k, k, M)] = conjc(r); rho2[I4(k, l, k, k, M)] = conjc(r);
+        }
+    for (int k = 0; k < M; k++)
+        for (int s = k + 1; s < M; s++) {
+            cplx r = R5[pair_index(k, s, M)];
+            rho2[I4(k, s, s, s, M)] = r; rho2[I4(s, k, s, s, M)] = r;
+            rho2[I4(s, s, s, k, M)] = conjc(r); rho2[I4(s, s, k, s, M)] = conjc(r);
+        }
+    for (int k = 0; k < M; k++)
+        for (int q = 0; q < M; q++)
+            for (int l = q + 1; l < M; l++) {
+                if (q == k || l == k) continue;
+                cplx r = R6[k * P + pair_index(q, l, M)];
+                rho2[I4(k, k, q, l, M)] = r; rho2[I4(k, k, l, q, M)] = r;
+                rho2[I4(l, q, k, k, M)] = conjc(r); rho2[I4(q, l, k, k, M)] = conjc(r);
+            }
+    for (int k = 0; k < M; k++)
+        for (int l = k + 1; l < M; l++)
+            for (int s = 0; s < M; s++) {
+                if (s == k || s == l) continue;
+                cplx r = R7[pair_index(k, l, M) * M + s];
+                rho2[I4(k, s, s, l, M)] = r; rho2[I4(s, k, s, l, M)] = r;
+                rho2[I4(s, k, l, s, M)] = r; rho2[I4(k, s, l, s, M)] = r;
+                rho2[I4(l, s, s, k, M)] = conjc(r); rho2[I4(s, l, s, k, M)] = conjc(r);
+                rho2[I4(s, l, k, s, M)] = conjc(r); rho2[I4(l, s, k, s, M)] = conjc(r);
+            }
+    for (int k = 0; k < M; k++)
+        for (int s = k + 1; s < M; s++)
+            for (int q = 0; q < M; q++)
+                for (int l = q + 1; l < M; l++) {
+                    if (q == k || q == s || l == k || l == s) continue;
+                    cplx r = tt[(size_t)pair_index(k, s, M) * P + pair_index(q, l, M)];
+                    rho2[I4(k, s, q, l, M)] = r; rho2[I4(s, k, q, l, M)] = r;
+                    rho2[I4(k, s, l, q, M)] = r; rho2[I4(s, k, l, q, M)] = r;
+                }
+    return MCPI_OK;
+}
+
+}  // namespace mcpi
+
+using namespace mcpi;
+
+extern "C" {
+
+mcpi_status mcpi_hip_rho1(mcpi_basis b, const mcpi_cplx *C_dev, mcpi_cplx *rho1)
+{
+    MCPI_ARG(b && C_dev && rho1, "null argument");
+    return densities(b, (const cplx *)C_dev, (cplx *)rho1, nullptr);
+}
+
+mcpi_status mcpi_hip_rho2(mcpi_basis b, const mcpi_cplx *C_dev, mcpi_cplx *rho2)
+{
+    MCPI_ARG(b && C_dev && rho2, "null argument");
+    return densities(b, (const cplx *)C_dev, nullptr, (cplx *)rho2);
+}
+
+mcpi_status mcpi_hip_vjp(mcpi_basis b, const mcpi_cplx *C_dev, mcpi_cplx *dE_dHo, mcpi_cplx *dE_dHint)
+{
+    MCPI_ARG(b && C_dev && dE_dHo && dE_dHint, "null argument");
+    MCPI_CHECK(densities(b, (const cplx *)C_dev, (cplx *)dE_dHo, (cplx *)dE_dHint));
+    // Hellmann-Feynman: E = sum Ho . rho1 + 1/2 sum Hint . rho2
+    const int64_t M4 = (int64_t)b->d.M * b->d.M * b->d.M * b->d.M;
+    cplx *g = (cplx *)dE_dHint;
+    for (int64_t x = 0; x < M4; x++) g[x] = cscale(g[x], 0.5);
+    return MCPI_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,144 @@
+"""H.v on the GPU against the oracle: every storage mode, zero-skipping on
+and off, edge-case bases, the reference-shaped and CUDA-shaped entry points,
+the golden fixtures, and the full N=12/M=12 and N=16/M=12 shapes through
+sampled rows and size-independent properties.  Tolerance: complex128 within
+1e-12 relative (normwise and elementwise against the largest entry)."""
+import glob
+import os
+
+import numpy as np
+import pytest
+
+import mcpi_amd
+import mcpi_amd.reference as R
+import oracle
+from tests.gpu_util import assert_close, note
+
+pytestmark = pytest.mark.gpu
+
+CASES = [(1, 1), (3, 1), (0, 3), (1, 3), (2, 3), (5, 5), (4, 6), (6, 4), (3, 7), (8, 5), (2, 9), (6, 9)]
+
+
+@pytest.mark.parametrize("N,M", CASES)
+@pytest.mark.parametrize("maps", ["full", "single", "none"])
+def test_apply_random(oracle_lib, N, M, maps):
+    note(f"acth random N={N} M={M} maps={maps}")
+    o = oracle.Basis(N, M)
+    C, Ho, Hint = oracle.random_inputs(N, M, seed=N * 100 + M)
+    ref = o.acth(C, Ho, Hint)
+    g = mcpi_amd.Basis(N, M, maps)
+    for skip in (True, False):
+        H = mcpi_amd.Hamiltonian(g, Ho, Hint, skip_zero=skip)
+        assert_close(H(C), ref, 1e-12, f"{maps} skip={skip}")
+
+
+@pytest.mark.parametrize("N,M,g", [(5, 11, 10.0), (8, 9, 0.3), (6, 12, 50.0)])
+@pytest.mark.parametrize("maps", ["full", "single", "none"])
+def test_apply_lieb_liniger(oracle_lib, N, M, g, maps):
+    note(f"acth LL N={N} M={M} g={g} maps={maps}")
+    o = oracle.Basis(N, M)
+    C, _, _ = oracle.random_inputs(N, M, seed=3)
+    Ho, Hint = oracle.ll_ho(M), oracle.ll_hint(M, g)
+    ref = o.acth(C, Ho, Hint)
+    b = mcpi_amd.Basis(N, M, maps)
+    for skip in (True, False):
+        assert_close(mcpi_amd.Hamiltonian(b, Ho, Hint, skip_zero=skip)(C), ref, 1e-12, f"skip={skip}")
+
+
+GOLD = sorted(glob.glob(os.path.join(os.path.dirname(__file__), "golden", "*.npz")))
+
+
+@pytest.mark.parametrize("path", GOLD, ids=[os.path.basename(p) for p in GOLD])
+def test_apply_golden(path):
+    gd = np.load(path)
+    N, M = int(gd["N"]), int(gd["M"])
+    note(f"acth golden {os.path.basename(path)}")
+    for maps in ("full", "single", "none"):
+        b = mcpi_amd.Basis(N, M, maps)
+        assert_close(mcpi_amd.Hamiltonian(b, gd["Ho"], gd["Hint"])(gd["C"]), gd["out"], 1e-12, maps)
+
+
+def test_reference_shaped_apply(oracle_lib):
+    for (N, M) in [(5, 5), (4, 6), (1, 4)]:
+        note(f"reference-shaped applyHconf_omp / applyHconf N={N} M={M}")
+        r = R.RefBasis(N, M)
+        o = oracle.Basis(N, M)
+        C, Ho, Hint = oracle.random_inputs(N, M, seed=5)
+        ref = o.acth(C, Ho, Hint)
+        assert_close(r.applyHconf_omp(C, Ho, Hint), ref, 1e-12, "applyHconf_omp")
+        assert_close(r.applyHconf_omp(C, Ho, Hint), ref, 1e-12, "cached tables")
+        assert_close(r.applyHconf(C, Ho, Hint), ref, 1e-12, "applyHconf")
+
+
+def test_cuda_shaped_dev_entry_point(oracle_lib):
+    N, M = 6, 5
+    note("applyHconf_dev N=6 M=5")
+    o = oracle.Basis(N, M)
+    C, Ho, Hint = oracle.random_inputs(N, M, seed=6)
+    D = mcpi_amd.DeviceArray.from_numpy
+    s_ot = np.concatenate([o.strideOT, [o.MapOT.size]]).astype(np.int64)
+    s_tt = np.concatenate([o.strideTT, [o.MapTT.size]]).astype(np.int64)
+    out = mcpi_amd.DeviceArray(o.nc)
+    mcpi_amd.applyHconf_dev(N, M, D(o.Map), D(o.MapOT), D(o.MapTT), D(s_ot), D(s_tt), D(o.IF),
+                            D(C), D(oracle.flat(Ho)), D(oracle.flat(Hint)), out)
+    assert_close(out.numpy(), o.acth(C, Ho, Hint), 1e-12)
+
+
+def _hermitian_check(H, nc, seed):
+    rng = np.random.default_rng(seed)
+    x = rng.standard_normal(nc) + 1j * rng.standard_normal(nc)
+    y = rng.standard_normal(nc) + 1j * rng.standard_normal(nc)
+    a = np.vdot(x, H(y))
+    c = np.vdot(y, H(x))
+    assert abs(a - np.conj(c)) <= 1e-11 * abs(a)
+
+
+@pytest.mark.parametrize("maps", ["full", "single"])
+def test_n12_m12_lieb_liniger_sampled_rows(oracle_lib, maps):
+    """N=12, M=12, Lieb-Liniger g=10: 4000 seeded rows recomputed by the
+    oracle's hash-only H.v; Hermiticity <x,Hy> = conj<y,Hx>."""
+    N, M = 12, 12
+    note(f"N=12 M=12 LL maps={maps}")
+    b = mcpi_amd.Basis(N, M, maps)
+    rng = np.random.default_rng(1212)
+    C = rng.standard_normal(b.nc) + 1j * rng.standard_normal(b.nc)
+    C /= np.linalg.norm(C)
+    Ho, Hint = oracle.ll_ho(M), oracle.ll_hint(M, 10.0)
+    H = mcpi_amd.Hamiltonian(b, Ho, Hint)
+    out = H(C)
+    rows = np.concatenate([[0, 1, b.nc - 1], rng.integers(0, b.nc, 4000)])
+    ref = oracle.acth_hash_rows(N, M, rows, C, Ho, Hint)
+    assert_close(out[rows], ref, 1e-12, "rows")
+    H2 = mcpi_amd.Hamiltonian(b, Ho, Hint, skip_zero=False)
+    assert_close(H2(C), out, 1e-12, "skip off")
+    _hermitian_check(H, b.nc, 7)
+
+
+def test_n12_m12_generic_sampled_rows(oracle_lib):
+    N, M = 12, 12
+    note("N=12 M=12 generic Hint, full maps")
+    b = mcpi_amd.Basis(N, M, "full")
+    _, Ho, Hint = oracle.random_inputs(3, M, seed=44, sigma=0.1)
+    rng = np.random.default_rng(5)
+    C = rng.standard_normal(b.nc) + 1j * rng.standard_normal(b.nc)
+    C /= np.linalg.norm(C)
+    H = mcpi_amd.Hamiltonian(b, Ho, Hint)
+    out = H(C)
+    rows = np.concatenate([[0, b.nc - 1], rng.integers(0, b.nc, 1500)])
+    assert_close(out[rows], oracle.acth_hash_rows(N, M, rows, C, Ho, Hint), 1e-12, "rows")
+
+
+def test_n16_m12_single_map_sampled_rows(oracle_lib):
+    """N=16, M=12, LL g=50, Map + on-the-fly double jumps (the large shape)."""
+    N, M = 16, 12
+    note("N=16 M=12 LL single map")
+    b = mcpi_amd.Basis(N, M, "single")
+    rng = np.random.default_rng(1612)
+    C = rng.standard_normal(b.nc) + 1j * rng.standard_normal(b.nc)
+    C /= np.linalg.norm(C)
+    Ho, Hint = oracle.ll_ho(M), oracle.ll_hint(M, 50.0)
+    H = mcpi_amd.Hamiltonian(b, Ho, Hint)
+    out = H(C)
+    rows = np.concatenate([[0, b.nc - 1], rng.integers(0, b.nc, 3000)])
+    assert_close(out[rows], oracle.acth_hash_rows(N, M, rows, C, Ho, Hint), 1e-12, "rows")
+    _hermitian_check(H, b.nc, 9)
