@@ -193,8 +193,13 @@ mcpi_status mcpi_hip_ham_destroy(mcpi_ham h);
  * for finite inputs; default on).  Off = every term of the storage mode is
  * evaluated and every map entry read, as the reference does. */
 mcpi_status mcpi_hip_ham_set_skip_zero(mcpi_ham h, int on);
-/* kernel schedule: 0 = auto (tuned), 1 = thread-per-row, 2 = tiled tasks */
+/* H.v kernel schedule: 0 = auto (the first H.v times the applicable
+ * schedules and keeps the fastest), 1 = generic thread-per-row kernel,
+ * 2 = kernel specialised on M reading every stored map, 3 = specialised
+ * kernel ranking double jumps on the fly even when the maps are stored */
 mcpi_status mcpi_hip_ham_set_schedule(mcpi_ham h, int schedule);
+/* schedule in use (after autotuning; 0 before the first H.v) */
+mcpi_status mcpi_hip_ham_get_schedule(mcpi_ham h, int *schedule);
 
 /* out = H C on device pointers (nc complex each), on stream (NULL = default) */
 mcpi_status mcpi_hip_applyH(mcpi_ham h, const mcpi_cplx *C_dev, mcpi_cplx *out_dev,

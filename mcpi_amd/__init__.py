@@ -91,9 +91,10 @@ def lib():
         "mcpi_hip_ham_destroy": (st, [_vp]),
         "mcpi_hip_ham_set_skip_zero": (st, [_vp, ct.c_int]),
         "mcpi_hip_ham_set_schedule": (st, [_vp, ct.c_int]),
+        "mcpi_hip_ham_get_schedule": (st, [_vp, ct.POINTER(ct.c_int)]),
         "mcpi_hip_applyH": (st, [_vp, _vp, _vp, _vp]),
         "mcpi_hip_applyH_timed": (st, [_vp, _vp, _vp, ct.c_int, _vp, ct.POINTER(ct.c_double)]),
-        "mcpi_hip_applyHconf_dev": (st, [ct.c_int, ct.c_int] + [_vp] * 11 + [_vp]),
+        "mcpi_hip_applyHconf_dev": (st, [ct.c_int, ct.c_int] + [_vp] * 11),
         "mcpi_hip_rho1": (st, [_vp, _vp, _c128p]),
         "mcpi_hip_rho2": (st, [_vp, _vp, _c128p]),
         "mcpi_hip_vjp": (st, [_vp, _vp, _c128p, _c128p]),
@@ -299,7 +300,15 @@ class Hamiltonian:
         _check(lib().mcpi_hip_ham_set_skip_zero(self.handle, int(bool(on))), "skip_zero")
 
     def set_schedule(self, schedule: int) -> None:
+        """0 auto-tuned, 1 generic, 2 specialised reading all maps, 3
+        specialised ranking double jumps on the fly."""
         _check(lib().mcpi_hip_ham_set_schedule(self.handle, int(schedule)), "schedule")
+
+    @property
+    def schedule(self) -> int:
+        s = ct.c_int(0)
+        _check(lib().mcpi_hip_ham_get_schedule(self.handle, ct.byref(s)), "get_schedule")
+        return s.value
 
     def apply(self, C: DeviceArray, out: DeviceArray, stream=None) -> None:
         _check(lib().mcpi_hip_applyH(self.handle, C.ptr, out.ptr, stream), "applyH")

@@ -84,9 +84,31 @@ mcpi_status build_coefficients(int M, const cplx *Ho, const cplx *Hint, HamCoefD
                     nzt[p * M2 + e] = nzc(a);
                 }
         }
-    // one device block: complex tables first (16-byte aligned), then masks
-    size_t nC = dk.size() + d1.size() + d2.size() + A0.size() + A.size() + ot.size() + tt.size();
-    size_t nB = nz1.size() + nzo.size() + nzt.size();
+    // zero-skipping lists: per-k single-jump masks and, per OT / TT chunk,
+    // the entries with a non-zero folded coefficient
+    std::vector<uint32_t> m1j(M, 0);
+    for (int k = 0; k < M; k++)
+        for (int l = 0; l < M; l++)
+            if (l != k && nz1[k + M * l]) m1j[k] |= 1u << l;
+    std::vector<int32_t> ot_off(M + 1), tt_off(std::max(P, 0) + 1), nz_e;
+    std::vector<cplx> nz_c;
+    for (int k = 0; k < M; k++) {
+        ot_off[k] = (int32_t)nz_e.size();
+        for (int e = 0; e < M2; e++)
+            if (nzo[k * M2 + e]) { nz_e.push_back(e); nz_c.push_back(ot[k * M2 + e]); }
+    }
+    ot_off[M] = (int32_t)nz_e.size();
+    for (int p = 0; p < P; p++) {
+        tt_off[p] = (int32_t)nz_e.size();
+        for (int e = 0; e < M2; e++)
+            if (nzt[p * M2 + e]) { nz_e.push_back(e); nz_c.push_back(tt[p * M2 + e]); }
+    }
+    tt_off[P] = (int32_t)nz_e.size();
+    if (nz_e.empty()) { nz_e.push_back(0); nz_c.push_back(cmake(0, 0)); }
+    // one device block: complex tables first (16-byte aligned), then integers
+    size_t nC = dk.size() + d1.size() + d2.size() + A0.size() + A.size() + ot.size() + tt.size() + nz_c.size();
+    size_t nB = nz1.size() + nzo.size() + nzt.size() + 4 * (m1j.size() + ot_off.size() + tt_off.size() +
+                                                           nz_e.size()) + 64;
     size_t bytes = nC * sizeof(cplx) + nB + 64;
     char *dev = nullptr;
     MCPI_HIP(hipMalloc(&dev, bytes));
@@ -100,6 +122,11 @@ mcpi_status build_coefficients(int M, const cplx *Ho, const cplx *Hint, HamCoefD
     size_t o_A = put(A.data(), A.size() * sizeof(cplx));
     size_t o_ot = put(ot.data(), ot.size() * sizeof(cplx));
     size_t o_tt = put(tt.data(), tt.size() * sizeof(cplx));
+    size_t o_nc = put(nz_c.data(), nz_c.size() * sizeof(cplx));
+    size_t o_m1 = put(m1j.data(), m1j.size() * 4);
+    size_t o_oo = put(ot_off.data(), ot_off.size() * 4);
+    size_t o_to = put(tt_off.data(), tt_off.size() * 4);
+    size_t o_ne = put(nz_e.data(), nz_e.size() * 4);
     size_t o_n1 = put(nz1.data(), nz1.size());
     size_t o_no = put(nzo.data(), nzo.size());
     size_t o_nt = put(nzt.data(), nzt.size());
@@ -118,6 +145,15 @@ mcpi_status build_coefficients(int M, const cplx *Ho, const cplx *Hint, HamCoefD
     c->nz_1j = (uint8_t *)(dev + o_n1);
     c->nz_ot = (uint8_t *)(dev + o_no);
     c->nz_tt = (uint8_t *)(dev + o_nt);
+    c->nz_c = (cplx *)(dev + o_nc);
+    c->m1j = (uint32_t *)(dev + o_m1);
+    c->ot_off = (int32_t *)(dev + o_oo);
+    c->tt_off = (int32_t *)(dev + o_to);
+    c->nz_e = (int32_t *)(dev + o_ne);
+    bool real = true;
+    for (const std::vector<cplx> *v : {&dk, &d1, &d2, &A0, &A, &ot, &tt})
+        for (const cplx &z : *v) real = real && z.im == 0.0;
+    c->real = real ? 1 : 0;
     *block = dev;
     return MCPI_OK;
 }
@@ -163,7 +199,9 @@ __global__ __launch_bounds__(256) void k_acth_rows(BasisDev b, HamCoefDev c, con
     for (int x = t; x < N + 3; x += T) sq[x] = sqrt((double)x);
     if (MODE < 2)
         for (int x = t; x < M * (N + 1); x += T) Bt[x] = b.B[x];
-    const int64_t i = (int64_t)blockIdx.x * T + t;
+    const unsigned per = gridDim.x / 8;   // XCD-contiguous row ranges (see acth_fast.h)
+    const unsigned bx = blockIdx.x < per * 8 ? (blockIdx.x % 8) * per + blockIdx.x / 8 : blockIdx.x;
+    const int64_t i = (int64_t)bx * T + t;
     const bool valid = i < nc;
     {
         int acc = 0;
@@ -277,10 +315,32 @@ static size_t rows_smem(int N, int M, int T)
     return sizeof(double) * (N + 3) + sizeof(int32_t) * M * (N + 1) + 2 * (size_t)M * T + 16;
 }
 
+mcpi_status launch_fast_a(const BasisDev &, int, const HamCoefDev &, int, int, const cplx *, cplx *, hipStream_t);
+mcpi_status launch_fast_b(const BasisDev &, int, const HamCoefDev &, int, int, const cplx *, cplx *, hipStream_t);
+mcpi_status launch_fast_c(const BasisDev &, int, const HamCoefDev &, int, int, const cplx *, cplx *, hipStream_t);
+
+// specialised kernel for b.M, or MCPI_ERR_STATE when there is none
+static mcpi_status launch_fast(const BasisDev &b, int maps, const HamCoefDev &c, int skip, int rank_doubles,
+                               const cplx *C, cplx *out, hipStream_t st)
+{
+    if (b.N > MCPI_MAXN) return MCPI_ERR_STATE;
+    if (b.M >= 3 && b.M <= 6) return launch_fast_a(b, maps, c, skip, rank_doubles, C, out, st);
+    if (b.M >= 7 && b.M <= 10) return launch_fast_b(b, maps, c, skip, rank_doubles, C, out, st);
+    if (b.M >= 11 && b.M <= 13) return launch_fast_c(b, maps, c, skip, rank_doubles, C, out, st);
+    return MCPI_ERR_STATE;
+}
+
+bool has_fast_kernel(int N, int M) { return N <= MCPI_MAXN && M >= 3 && M <= 13; }
+
 mcpi_status launch_apply_raw(const BasisDev &b, int maps, const HamCoefDev &c, int skip_zero,
                              int schedule, const cplx *C, cplx *out, hipStream_t st)
 {
-    (void)schedule;
+    if (schedule == 2 || schedule == 3) {
+        mcpi_status s = launch_fast(b, maps, c, skip_zero, schedule == 3, C, out, st);
+        if (s != MCPI_ERR_STATE) return s;
+        set_error("no specialised H.v kernel for this number of orbitals");
+        return MCPI_ERR_ARG;
+    }
     const int T = 256;
     dim3 grid((unsigned)((b.nc + T - 1) / T));
     size_t sm = rows_smem(b.N, b.M, T);
@@ -294,10 +354,58 @@ mcpi_status launch_apply_raw(const BasisDev &b, int maps, const HamCoefDev &c, i
     return MCPI_OK;
 }
 
+// Autotuner: the first H.v of a Hamiltonian (per zero-skipping setting, and
+// outside graph capture) times every applicable schedule with HIP events on
+// the caller's stream and keeps the fastest.  All schedules compute the same
+// result; the last one timed leaves its output in `out`.
+static mcpi_status autotune(mcpi_ham h, const cplx *C, cplx *out, hipStream_t st)
+{
+    const BasisDev &b = h->basis->d;
+    const int maps = h->basis->maps;
+    int cands[3], n = 0;
+    cands[n++] = 1;
+    if (has_fast_kernel(b.N, b.M)) {
+        cands[n++] = 2;
+        if (maps == MCPI_MAPS_FULL) cands[n++] = 3;
+    }
+    int best = cands[n - 1];
+    if (n > 1) {
+        hipEvent_t e0, e1;
+        MCPI_HIP(hipEventCreate(&e0));
+        MCPI_HIP(hipEventCreate(&e1));
+        float best_ms = 1e30f;
+        for (int x = 0; x < n; x++) {
+            MCPI_CHECK(launch_apply_raw(b, maps, h->c, h->skip_zero, cands[x], C, out, st));  // warm-up
+            MCPI_HIP(hipEventRecord(e0, st));
+            MCPI_CHECK(launch_apply_raw(b, maps, h->c, h->skip_zero, cands[x], C, out, st));
+            MCPI_HIP(hipEventRecord(e1, st));
+            MCPI_HIP(hipEventSynchronize(e1));
+            float ms = 0;
+            MCPI_HIP(hipEventElapsedTime(&ms, e0, e1));
+            if (ms < best_ms) { best_ms = ms; best = cands[x]; }
+        }
+        MCPI_HIP(hipEventDestroy(e0));
+        MCPI_HIP(hipEventDestroy(e1));
+    }
+    h->tuned[h->skip_zero ? 1 : 0] = best;
+    return MCPI_OK;
+}
+
 mcpi_status launch_apply(mcpi_ham h, const cplx *C, cplx *out, hipStream_t st)
 {
-    return launch_apply_raw(h->basis->d, h->basis->maps, h->c, h->skip_zero,
-                            h->schedule ? h->schedule : h->tuned_schedule, C, out, st);
+    int sched = h->schedule;
+    if (sched == 0) {
+        int &t = h->tuned[h->skip_zero ? 1 : 0];
+        if (t == 0) {
+            hipStreamCaptureStatus cs = hipStreamCaptureStatusNone;
+            MCPI_HIP(hipStreamIsCapturing(st, &cs));
+            if (cs == hipStreamCaptureStatusNone) MCPI_CHECK(autotune(h, C, out, st));
+        }
+        sched = t ? t : (has_fast_kernel(h->basis->d.N, h->basis->d.M) ? 2 : 1);
+    }
+    return launch_apply_raw(h->basis->d, h->basis->maps, h->c, h->skip_zero, sched, C, out, st);
 }
+
+int ham_schedule(mcpi_ham h) { return h->schedule ? h->schedule : h->tuned[h->skip_zero ? 1 : 0]; }
 
 }  // namespace mcpi

@@ -1,0 +1,411 @@
+// H.v kernels specialised on the number of orbitals M.
+//
+// Same maths as the generic kernel in acth.hip (reference applyHconf_omp,
+// hamiltonianMatrix.h:1431-1970, folded into per-target coefficient tables).
+// With M known at compile time a row's occupations, square roots and prefix
+// counts live in registers, and the loops over orbitals are uniform across
+// the wavefront: coefficients are scalar loads, and the terms of one chunk
+// form a single basic block, so the loads of map entries and the gathers
+// C[j] of a whole chunk are issued together instead of one term at a time.
+//
+// Lanes are consecutive rows: the terms a wavefront evaluates together
+// gather C[j] from neighbouring targets (about four gathers per 128-byte
+// line at N = M = 12).
+//
+// Template parameters:
+//   MAP1  single jumps from the stored Map (else ranked on the fly)
+//   DMAP  double jumps from the stored MapOT/MapTT (else ranked on the fly)
+//   SKIP  terms whose folded coefficient is exactly zero are not evaluated:
+//         the kernel walks per-chunk lists of the non-zero terms instead of
+//         the whole chunk (result-neutral for finite inputs)
+//
+// Reading a double-jump chunk (DMAP): row l of a chunk holds entries
+// l*M .. l*M + l contiguously (the q <= l half actH uses); they are loaded as
+// whole 16-byte pieces, so every cache line is requested by consecutive
+// loads of one lane and fetched once.
+//
+// Ranking on the fly: with prefix counts P_j = sum_{o<j} v_o,
+// rank(v) = nc - 1 - sum_{j=1}^{M-1} B[j][P_j] (closed form of the
+// FockToIndex loop, configurationsMap.h:273-305).  Removing particles from a
+// uniform set of orbitals shifts P_j by -A_j (uniform) and adding them at
+// (q, l) by Q_j = [q<j] + [l<j], a step function of j.  With
+// U_a[x] = sum_{j=1}^{x} B[j][P_j - A_j + a] (a = 0, 1, 2) the target is
+//   rank = nc - 1 - (U0[q] + U1[l] - U1[q] + U2[M-1] - U2[l])   (q < l)
+//   rank = nc - 1 - (U0[q] + U2[M-1] - U2[q])                   (q == l)
+// and with D01 = U0 - U1, D12 = U1 - U2, T = U2[M-1] both cases read
+//   rank = nc - 1 - T - D01[q] - D12[l],
+// so per chunk each lane builds two M-entry tables and every term costs two
+// adds (a single jump k -> l: rank = nc - 1 - U1[M-1] - D01[l]).  Arithmetic
+// is modulo 2^32 (exact: the result lies in [0, nc) and nc < 2^31).
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+
+#include "mcpi_internal.h"
+
+namespace mcpi {
+
+__device__ inline cplx ld_c(const cplx *p)
+{
+    double2 t = *reinterpret_cast<const double2 *>(p);
+    return cmake(t.x, t.y);
+}
+
+// 16-byte piece of a chunk; dword alignment suffices for global_load_dwordx4
+struct __attribute__((aligned(4))) Piece4 {
+    int32_t x[4];
+};
+
+// acc += coef * (x * c) with a real scale x; RC: coef is known to be real
+template <bool RC>
+__device__ inline cplx term(cplx coef, double x, cplx c, cplx acc)
+{
+    if (RC) {
+        const double w = coef.re * x;
+        acc.re = fma(w, c.re, acc.re);
+        acc.im = fma(w, c.im, acc.im);
+    } else {
+        const double wr = coef.re * x, wi = coef.im * x;
+        acc.re = fma(wr, c.re, acc.re);
+        acc.re = fma(-wi, c.im, acc.re);
+        acc.im = fma(wr, c.im, acc.im);
+        acc.im = fma(wi, c.re, acc.im);
+    }
+    return acc;
+}
+
+// Workgroups are dealt to the 8 XCDs round-robin; remap them so that each
+// XCD (with its own L2) processes one contiguous range of rows, whose gather
+// targets cluster, instead of every eighth tile of the whole basis.
+__device__ inline unsigned xcd_block(unsigned b, unsigned n)
+{
+    const unsigned per = n / 8;
+    if (b >= per * 8) return b;
+    return (b % 8) * per + b / 8;
+}
+
+// Per-row state kept in registers: the occupations packed 8 bits per orbital
+// (M <= 16) and sqrt(v + 1).  Everything else is derived on use (prefix
+// counts inside the rank tables, sqrt(v) and sqrt(v + 2) from the LDS table),
+// which keeps the whole-chunk basic blocks within 128 VGPRs.
+template <int M>
+struct RowState {
+    uint64_t pk[2];
+    double r1[M];   // sqrt(v+1)
+};
+
+// occupation of orbital k (compile-time or uniform runtime k)
+template <int M>
+__device__ inline int occ(const RowState<M> &r, int k)
+{
+    return (int)((k < 8 ? (r.pk[0] >> (8 * k)) : (r.pk[1] >> (8 * (k - 8)))) & 0xff);
+}
+
+template <int M>
+__device__ inline void load_row(const BasisDev &b, int64_t i, const double *sq, RowState<M> &r)
+{
+    r.pk[0] = r.pk[1] = 0;
+#pragma unroll
+    for (int j = 0; j < M; j++) {
+        const int v = b.IF[i * M + j];
+        r.r1[j] = sq[v + 1];
+        if (j < 8) r.pk[0] |= (uint64_t)v << (8 * j);
+        else r.pk[1] |= (uint64_t)v << (8 * (j - 8));
+    }
+}
+
+// Difference tables of the prefix sums U_a (see the header comment) for the
+// removal profile A_j = [ra<j] + [rb<j] (ra, rb uniform; rb = -1 removes a
+// single particle).  DBL: the move adds two particles (tables D01, D12, total
+// U2[M-1]); otherwise one (D01 and total U1[M-1]).  Table indices are clamped:
+// out-of-range entries only enter differences that cancel.
+template <int M, bool DBL>
+__device__ inline void build_d(const RowState<M> &r, const uint32_t *Bt, int N, int ra, int rb,
+                               uint32_t (&D0)[M], uint32_t (&D1)[M], uint32_t &tot)
+{
+    uint32_t s0 = 0, s1 = 0, s2 = 0;
+    int P = 0;   // prefix count sum_{o<j} v_o
+    D0[0] = D1[0] = 0;
+#pragma unroll
+    for (int j = 1; j < M; j++) {
+        P += occ<M>(r, j - 1);
+        const int p = P - (ra < j) - (rb >= 0 && rb < j);
+        const uint32_t *row = Bt + j * (N + 1);
+        const int p0 = p < 0 ? 0 : (p > N ? N : p);
+        const int p1 = p + 1 < 0 ? 0 : (p + 1 > N ? N : p + 1);
+        s0 += row[p0];
+        s1 += row[p1];
+        if (DBL) {
+            const int p2 = p + 2 < 0 ? 0 : (p + 2 > N ? N : p + 2);
+            s2 += row[p2];
+        }
+        D0[j] = s0 - s1;
+        D1[j] = DBL ? s1 - s2 : 0u;
+    }
+    tot = DBL ? s2 : s1;
+}
+
+// per-lane copy of the difference tables in LDS (layout [a][x][lane]) for
+// lookups at runtime orbitals: register arrays indexed at runtime would be
+// demoted to scratch memory
+template <int M>
+__device__ inline void store_d(const uint32_t (&D0)[M], const uint32_t (&D1)[M], uint32_t *Dl, int lane, int T)
+{
+#pragma unroll
+    for (int x = 0; x < M; x++) {
+        Dl[x * T + lane] = D0[x];
+        Dl[(M + x) * T + lane] = D1[x];
+    }
+}
+
+template <int M>
+__device__ inline uint32_t dl(const uint32_t *Dl, int lane, int T, int a, int x)
+{
+    return Dl[(a * M + x) * T + lane];
+}
+
+// ------------------------------------------------------- chunk bodies --
+
+// every entry q <= l of one chunk (coefficients of excluded entries are
+// zero; their targets are valid states, so no branch is needed).  Row l:
+// sum_{q<l} coef r1[q] C[j] is scaled once by r1[l]; q == l uses r2[l].
+template <int M, bool DMAP, bool RC>
+__device__ inline cplx chunk_all(const int32_t *__restrict__ chunk, const uint32_t (&D0)[M],
+                                 const uint32_t (&D1)[M], uint32_t base, const cplx *__restrict__ C,
+                                 const RowState<M> &r, const double *sq, const cplx *coef)
+{
+    cplx part = cmake(0, 0);
+#pragma unroll
+    for (int l = 0; l < M; l++) {
+        const int e0 = l * M;
+        const int p0 = e0 / 4, p1 = (e0 + l) / 4;
+        int32_t val[8 * 4];
+        if (DMAP) {
+#pragma unroll
+            for (int pc = p0; pc <= p1; pc++) {
+                Piece4 t = *reinterpret_cast<const Piece4 *>(chunk + 4 * pc);
+#pragma unroll
+                for (int u = 0; u < 4; u++) val[4 * (pc - p0) + u] = t.x[u];
+            }
+        }
+        cplx pl = cmake(0, 0);
+#pragma unroll
+        for (int q = 0; q <= l; q++) {
+            const int e = e0 + q;
+            const int64_t j = DMAP ? (int64_t)val[e - 4 * p0] : (int64_t)(uint32_t)(base - D0[q] - D1[l]);
+            const cplx cj = ld_c(C + j);
+            if (q < l) pl = term<RC>(coef[e], r.r1[q], cj, pl);
+            else part = term<RC>(coef[e], r.r1[q] * sq[occ<M>(r, q) + 2], cj, part);
+        }
+        part.re = fma(r.r1[l], pl.re, part.re);
+        part.im = fma(r.r1[l], pl.im, part.im);
+    }
+    return part;
+}
+
+// the non-zero entries of one chunk from a uniform list (four at a time);
+// q and l are runtime values here, so occupations come from the packed copy,
+// square roots from the LDS table and prefix differences from the LDS copy
+template <int M, bool DMAP, bool RC>
+__device__ inline cplx chunk_list(const int32_t *__restrict__ chunk, const uint32_t *Dl, int lane, int T,
+                                  uint32_t base, const cplx *__restrict__ C, const RowState<M> &r,
+                                  const double *sq, const int32_t *ents, const cplx *coefs, int n)
+{
+    cplx part = cmake(0, 0);
+    for (int t = 0; t < n; t += 4) {
+        int64_t j[4];
+        double x[4];
+        cplx cf[4];
+#pragma unroll
+        for (int u = 0; u < 4; u++) {
+            const bool ok = t + u < n;  // uniform
+            const int e = ok ? ents[t + u] : 0;
+            const int q = e % M, l = e / M;
+            if (DMAP) j[u] = chunk[e];
+            else j[u] = (int64_t)(uint32_t)(base - dl<M>(Dl, lane, T, 0, q) - dl<M>(Dl, lane, T, 1, l));
+            const int vq = occ<M>(r, q), vl = occ<M>(r, l);
+            x[u] = ok ? ((q == l) ? sq[vq + 1] * sq[vq + 2] : sq[vq + 1] * sq[vl + 1]) : 0.0;
+            cf[u] = ok ? coefs[t + u] : cmake(0, 0);
+        }
+#pragma unroll
+        for (int u = 0; u < 4; u++) part = term<RC>(cf[u], x[u], ld_c(C + j[u]), part);
+    }
+    return part;
+}
+
+// ---------------------------------------------------------------- kernel --
+
+// Occupancy: the zero-skipping kernels fit in 128 VGPRs (4 waves per SIMD);
+// the every-term kernels keep a whole chunk's gathers in flight and are
+// given up to 256 VGPRs (2 waves per SIMD) rather than spill.
+template <int M, bool MAP1, bool DMAP, bool SKIP, bool RC>
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(SKIP ? 4 : 2))) void k_acth_fast(
+    BasisDev b, HamCoefDev c, const cplx *__restrict__ C, cplx *__restrict__ out)
+{
+    constexpr int MM = M * M;
+    extern __shared__ double fsm[];
+    double *sq = fsm;                                   // N+3
+    uint32_t *Bt = (uint32_t *)(sq + b.N + 3);          // M*(N+1)
+    uint32_t *Dl = Bt + M * (b.N + 1);                  // 2*M*blockDim (SKIP with ranking only)
+    const int lane = threadIdx.x, T = blockDim.x;
+    for (int x = threadIdx.x; x < b.N + 3; x += blockDim.x) sq[x] = sqrt((double)x);
+    if (!MAP1 || !DMAP)
+        for (int x = threadIdx.x; x < M * (b.N + 1); x += blockDim.x) Bt[x] = (uint32_t)b.B[x];
+    __syncthreads();
+    const int64_t i = (int64_t)xcd_block(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
+    if (i >= b.nc) return;
+    const int N = b.N;
+    const int64_t nc = b.nc;
+    const uint32_t top = (uint32_t)(nc - 1);
+    RowState<M> r;
+    load_row<M>(b, i, sq, r);
+
+    // diagonal: 1-body diagonal, rules 1 and 2
+    cplx dsum = cmake(0, 0);
+#pragma unroll
+    for (int k = 0; k < M; k++) {
+        const int ik = occ<M>(r, k);
+        const double vk = (double)ik;
+        dsum = cfma(c.dk[k], cmake(vk, 0), dsum);
+        dsum = cfma(c.d1[k], cmake(vk * (ik - 1), 0), dsum);
+#pragma unroll
+        for (int s = k + 1; s < M; s++) dsum = cfma(c.d2[k + M * s], cmake(vk * occ<M>(r, s), 0), dsum);
+    }
+    cplx acc = cfma(dsum, ld_c(C + i), cmake(0, 0));
+
+    // single jumps k -> l: 1-body off-diagonal, rules 4, 5, 8
+    for (int k = 0; k < M; k++) {
+        const uint32_t lmask = c.m1j[k];   // l with a non-zero coefficient
+        if (SKIP && lmask == 0) continue;
+        const int vk = occ<M>(r, k);
+        const bool act = vk >= 1;
+        uint32_t D0[M], D1[M], tot = 0;
+        if (!MAP1) build_d<M, false>(r, Bt, N, k, -1, D0, D1, tot);
+        if (!MAP1 && SKIP) store_d<M>(D0, D1, Dl, lane, T);
+        const uint32_t base = top - tot;
+        const double fk = act ? sq[vk] : 0.0;
+        if (!SKIP) {
+#pragma unroll
+            for (int l = 0; l < M; l++) {
+                const int kl = k + M * l;   // the l == k entry has zero coefficients
+                cplx h = c.A0[kl];
+                const cplx *Ak = c.A + (int64_t)kl * M;
+#pragma unroll
+                for (int s = 0; s < M; s++) {
+                    h.re = fma((double)occ<M>(r, s), Ak[s].re, h.re);
+                    h.im = fma((double)occ<M>(r, s), Ak[s].im, h.im);
+                }
+                int64_t j;
+                if (MAP1) j = b.Map[i + (int64_t)kl * nc];
+                else j = (int64_t)(uint32_t)(base - D0[l]);
+                j = act ? j : i;
+                acc = term<RC>(h, fk * r.r1[l], ld_c(C + j), acc);
+            }
+        } else {
+            uint32_t m = lmask;
+            while (m) {
+                const int l = __ffs(m) - 1;
+                m &= m - 1;
+                const int kl = k + M * l;
+                cplx h = c.A0[kl];
+                const cplx *Ak = c.A + (int64_t)kl * M;
+#pragma unroll
+                for (int s = 0; s < M; s++) {
+                    h.re = fma((double)occ<M>(r, s), Ak[s].re, h.re);
+                    h.im = fma((double)occ<M>(r, s), Ak[s].im, h.im);
+                }
+                int64_t j;
+                if (MAP1) j = act ? b.Map[i + (int64_t)kl * nc] : i;
+                else j = act ? (int64_t)(uint32_t)(base - dl<M>(Dl, lane, T, 0, l)) : i;
+                acc = term<RC>(h, fk * sq[occ<M>(r, l) + 1], ld_c(C + j), acc);
+            }
+        }
+    }
+
+    // same-orbital double jumps: rules 3, 6
+    {
+        const int32_t *sect = DMAP ? b.MapOT + b.strideOT[i] : nullptr;
+        int cnt = 0;
+        for (int k = 0; k < M; k++) {
+            const int n_nz = SKIP ? c.ot_off[k + 1] - c.ot_off[k] : 1;
+            const int vk = occ<M>(r, k);
+            const bool act = vk >= 2;
+            const int32_t *chunk = DMAP ? sect + (int64_t)cnt * MM : nullptr;
+            cnt += act;
+            if (SKIP && n_nz == 0) continue;
+            if (!act) continue;
+            uint32_t D0[M], D1[M], tot = 0;
+            if (!DMAP) build_d<M, true>(r, Bt, N, k, k, D0, D1, tot);
+            if (!DMAP && SKIP) store_d<M>(D0, D1, Dl, lane, T);
+            const uint32_t base = top - tot;
+            const double f = sq[vk] * sq[vk - 1];
+            cplx part;
+            if (!SKIP) part = chunk_all<M, DMAP, RC>(chunk, D0, D1, base, C, r, sq, c.ot + (int64_t)k * MM);
+            else
+                part = chunk_list<M, DMAP, RC>(chunk, Dl, lane, T, base, C, r, sq, c.nz_e + c.ot_off[k],
+                                               c.nz_c + c.ot_off[k], n_nz);
+            acc.re = fma(f, part.re, acc.re);
+            acc.im = fma(f, part.im, acc.im);
+        }
+    }
+    // two-orbital double jumps: rules 7, 9
+    {
+        const int32_t *sect = DMAP ? b.MapTT + b.strideTT[i] : nullptr;
+        int cnt = 0, p = 0;
+        for (int k = 0; k < M; k++) {
+            const int vk = occ<M>(r, k);
+            for (int s = k + 1; s < M; s++, p++) {
+                const int n_nz = SKIP ? c.tt_off[p + 1] - c.tt_off[p] : 1;
+                const int vs = occ<M>(r, s);
+                const bool act = vk >= 1 && vs >= 1;
+                const int32_t *chunk = DMAP ? sect + (int64_t)cnt * MM : nullptr;
+                cnt += act;
+                if (SKIP && n_nz == 0) continue;
+                if (!act) continue;
+                uint32_t D0[M], D1[M], tot = 0;
+                if (!DMAP) build_d<M, true>(r, Bt, N, k, s, D0, D1, tot);
+                if (!DMAP && SKIP) store_d<M>(D0, D1, Dl, lane, T);
+                const uint32_t base = top - tot;
+                const double f = sq[vk] * sq[vs];
+                cplx part;
+                if (!SKIP) part = chunk_all<M, DMAP, RC>(chunk, D0, D1, base, C, r, sq, c.tt + (int64_t)p * MM);
+                else
+                    part = chunk_list<M, DMAP, RC>(chunk, Dl, lane, T, base, C, r, sq, c.nz_e + c.tt_off[p],
+                                                   c.nz_c + c.tt_off[p], n_nz);
+                acc.re = fma(f, part.re, acc.re);
+                acc.im = fma(f, part.im, acc.im);
+            }
+        }
+    }
+    double2 o;
+    o.x = acc.re;
+    o.y = acc.im;
+    *reinterpret_cast<double2 *>(out + i) = o;
+}
+
+// launch one specialisation; rank_doubles: rank double jumps even when the
+// maps are stored; real: every folded coefficient has a zero imaginary part
+template <int M>
+inline mcpi_status launch_fast_m(const BasisDev &b, int maps, const HamCoefDev &c, int skip, int rank_doubles,
+                                 const cplx *C, cplx *out, hipStream_t st)
+{
+    dim3 grid((unsigned)((b.nc + 255) / 256)), blk(256);
+    size_t sm = sizeof(double) * (b.N + 3) + sizeof(uint32_t) * M * (b.N + 1) + 16;
+    if (skip) sm += sizeof(uint32_t) * 2 * M * 256;
+    const bool map1 = maps >= MCPI_MAPS_SINGLE;
+    const bool dmap = maps == MCPI_MAPS_FULL && !rank_doubles;
+    const bool real = c.real;
+#define MCPI_L(A, B, S, R) hipLaunchKernelGGL((k_acth_fast<M, A, B, S, R>), grid, blk, sm, st, b, c, C, out)
+#define MCPI_L2(A, B, S) { if (real) MCPI_L(A, B, S, true); else MCPI_L(A, B, S, false); }
+    if (map1 && dmap) { if (skip) MCPI_L2(true, true, true) else MCPI_L2(true, true, false) }
+    else if (map1) { if (skip) MCPI_L2(true, false, true) else MCPI_L2(true, false, false) }
+    else { if (skip) MCPI_L2(false, false, true) else MCPI_L2(false, false, false) }
+#undef MCPI_L2
+#undef MCPI_L
+    MCPI_HIP(hipGetLastError());
+    return MCPI_OK;
+}
+
+}  // namespace mcpi

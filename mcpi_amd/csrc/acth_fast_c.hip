@@ -1,0 +1,17 @@
+// Instantiations of the specialised H.v kernels (acth_fast.h) for M in {11 12 13}.
+#include "acth_fast.h"
+
+namespace mcpi {
+
+mcpi_status launch_fast_c(const BasisDev &b, int maps, const HamCoefDev &c, int skip, int rank_doubles,
+                             const cplx *C, cplx *out, hipStream_t st)
+{
+    switch (b.M) {
+    case 11: return launch_fast_m<11>(b, maps, c, skip, rank_doubles, C, out, st);
+    case 12: return launch_fast_m<12>(b, maps, c, skip, rank_doubles, C, out, st);
+    case 13: return launch_fast_m<13>(b, maps, c, skip, rank_doubles, C, out, st);
+    default: return MCPI_ERR_STATE;
+    }
+}
+
+}  // namespace mcpi

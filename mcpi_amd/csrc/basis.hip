@@ -300,7 +300,9 @@ static mcpi_status scan_to_strides(int64_t n, const int32_t *cnt, int64_t factor
     hipLaunchKernelGGL(k_scan_final, dim3(nb), dim3(SCAN_BLOCK), 0, 0, n, cnt, partial, factor, out);
     hipLaunchKernelGGL(k_scan_last, dim3(1), dim3(1), 0, 0, n, grand, factor, out);
     MCPI_HIP(hipGetLastError());
-    MCPI_HIP(hipMemcpy(total_host, grand, sizeof(int64_t), hipMemcpyDeviceToHost));
+    int64_t chunks = 0;
+    MCPI_HIP(hipMemcpy(&chunks, grand, sizeof(int64_t), hipMemcpyDeviceToHost));
+    *total_host = chunks * factor;   // table length, matching out[n]
     MCPI_HIP(hipFree(partial));
     MCPI_HIP(hipFree(grand));
     return MCPI_OK;
@@ -329,9 +331,12 @@ static void free_basis(mcpi_basis b)
     delete b;
 }
 
+// Every table gets 64 bytes of padding: the specialised H.v kernels read
+// map chunks in 16-byte pieces, and for odd M the last piece of the last
+// chunk extends up to 12 bytes past the table.
 static mcpi_status alloc_dev(void **p, int64_t bytes, int64_t *acct)
 {
-    hipError_t e = hipMalloc(p, (size_t)std::max<int64_t>(bytes, 16));
+    hipError_t e = hipMalloc(p, (size_t)std::max<int64_t>(bytes, 0) + 64);
     if (e != hipSuccess) {
         (void)hipGetLastError();
         char msg[160];
@@ -504,7 +509,32 @@ mcpi_status basis_build_parts(mcpi_basis b, int want_map, int want_ot, int want_
 
 }  // namespace mcpi
 
+// streaming copy, 16 bytes per lane (also a kernel of exactly known traffic)
+__global__ __launch_bounds__(256) void k_copy16(int64_t n16, const double2 *__restrict__ src,
+                                                double2 *__restrict__ dst)
+{
+    for (int64_t x = (int64_t)blockIdx.x * 256 + threadIdx.x; x < n16; x += (int64_t)gridDim.x * 256)
+        dst[x] = src[x];
+}
+
 extern "C" {
+
+mcpi_status mcpi_hip_memcpy_d2d(void *dst, const void *src, size_t bytes)
+{
+    MCPI_ARG(dst && src, "null pointer");
+    const uintptr_t a = (uintptr_t)dst | (uintptr_t)src | (uintptr_t)bytes;
+    if (bytes && (a & 15) == 0) {
+        int64_t n16 = bytes / 16;
+        int64_t grid = std::min<int64_t>((n16 + 255) / 256, 8192);
+        hipLaunchKernelGGL(k_copy16, dim3((unsigned)grid), dim3(256), 0, 0, n16, (const double2 *)src,
+                           (double2 *)dst);
+        MCPI_HIP(hipGetLastError());
+        MCPI_HIP(hipDeviceSynchronize());
+    } else if (bytes) {
+        MCPI_HIP(hipMemcpy(dst, src, bytes, hipMemcpyDeviceToDevice));
+    }
+    return MCPI_OK;
+}
 
 mcpi_status mcpi_hip_basis_create(int N, int M, int maps, mcpi_basis *out)
 {

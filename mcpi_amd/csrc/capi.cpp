@@ -75,11 +75,6 @@ mcpi_status mcpi_hip_memcpy_d2h(void *dst, const void *src, size_t bytes)
     MCPI_HIP(hipMemcpy(dst, src, bytes, hipMemcpyDeviceToHost));
     return MCPI_OK;
 }
-mcpi_status mcpi_hip_memcpy_d2d(void *dst, const void *src, size_t bytes)
-{
-    MCPI_HIP(hipMemcpy(dst, src, bytes, hipMemcpyDeviceToDevice));
-    return MCPI_OK;
-}
 mcpi_status mcpi_hip_memset(void *dptr, int value, size_t bytes)
 {
     MCPI_HIP(hipMemset(dptr, value, bytes));
@@ -113,7 +108,7 @@ mcpi_status mcpi_hip_ham_create(mcpi_basis b, const mcpi_cplx *Ho, const mcpi_cp
     h->basis = b;
     h->skip_zero = 1;
     h->schedule = 0;
-    h->tuned_schedule = 1;
+    h->tuned[0] = h->tuned[1] = 0;
     h->Ho_dev = h->Hint_dev = nullptr;
     mcpi_status st = build_coefficients(M, (const cplx *)Ho, (const cplx *)Hint, &h->c, &h->coef_block);
     if (st == MCPI_OK) {
@@ -153,8 +148,15 @@ mcpi_status mcpi_hip_ham_set_skip_zero(mcpi_ham h, int on)
 mcpi_status mcpi_hip_ham_set_schedule(mcpi_ham h, int schedule)
 {
     MCPI_ARG(h, "null handle");
-    MCPI_ARG(schedule >= 0 && schedule <= 2, "schedule must be 0, 1 or 2");
+    MCPI_ARG(schedule >= 0 && schedule <= 3, "schedule must be 0 (auto), 1, 2 or 3");
     h->schedule = schedule;
+    return MCPI_OK;
+}
+
+mcpi_status mcpi_hip_ham_get_schedule(mcpi_ham h, int *schedule)
+{
+    MCPI_ARG(h && schedule, "null argument");
+    *schedule = mcpi::ham_schedule(h);
     return MCPI_OK;
 }
 

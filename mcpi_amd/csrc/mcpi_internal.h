@@ -123,6 +123,13 @@ struct HamCoefDev {
     uint8_t *nz_1j;  // M*M: 1 if any coefficient of the jump k->l is non-zero
     uint8_t *nz_ot;  // M*M*M: per (k, e)
     uint8_t *nz_tt;  // P*M*M: per (p, e)
+    // zero-skipping support for the specialised kernels
+    uint32_t *m1j;   // M: bit l set if the jump k->l has a non-zero coefficient
+    int32_t *ot_off; // M+1: non-zero list of OT chunk k is nz_e/nz_c[ot_off[k] .. ot_off[k+1])
+    int32_t *tt_off; // P+1: same for TT chunk p
+    int32_t *nz_e;   // entry index q + l*M of each listed term
+    cplx *nz_c;      // its folded coefficient
+    int real;        // 1 if every coefficient above has a zero imaginary part
 };
 
 struct mcpi_ham_s {
@@ -133,7 +140,7 @@ struct mcpi_ham_s {
     cplx *Hint_dev;     // M^4
     int skip_zero;
     int schedule;
-    int tuned_schedule;
+    int tuned[2];       // autotuned schedule per skip_zero value (0 = not tuned)
 };
 
 namespace mcpi {
@@ -146,6 +153,9 @@ __host__ __device__ inline int pair_index(int k, int s, int M)
 
 // kernels launched from host code of other translation units
 mcpi_status launch_apply(mcpi_ham h, const cplx *C, cplx *out, hipStream_t st);
+int ham_schedule(mcpi_ham h);
+// schedules: 1 generic thread-per-row kernel, 2 specialised kernel reading
+// every stored map, 3 specialised kernel ranking double jumps on the fly
 mcpi_status launch_apply_raw(const BasisDev &b, int maps, const HamCoefDev &c, int skip_zero,
                              int schedule, const cplx *C, cplx *out, hipStream_t st);
 mcpi_status build_coefficients(int M, const cplx *Ho, const cplx *Hint, HamCoefDev *c,

@@ -68,6 +68,8 @@ def lib():
         "orc_two_two_map": (None, [_int, _int, _i32p, _i32p, _i64p, _i32p]),
         "orc_acth": (None, [_int, _int, _i32p, _i32p, _i32p, _i64p, _i64p, _i32p,
                             _c128p, _c128p, _c128p, _c128p]),
+        "orc_acth_range": (None, [_int, _int, _i64, _i64, _i32p, _i32p, _i32p, _i64p, _i64p,
+                                  _i32p, _c128p, _c128p, _c128p, _c128p]),
         "orc_acth_hash_rows": (None, [_int, _int, _i32p, _i64, _i64p, _c128p,
                                       _c128p, _c128p, _c128p]),
         "orc_rho1": (None, [_int, _int, _i32p, _i32p, _c128p, _c128p]),

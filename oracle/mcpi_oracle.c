@@ -307,24 +307,26 @@ static inline int64_t tt_ordinal(const int32_t *v, int M, int k, int s)
     return c;
 }
 
-/* Action of H on C with all three maps.  Restates applyHconf_omp,
+/* Action of H on C with all three maps for rows [row0, row1) (out_rows[i -
+ * row0]).  Restates applyHconf_omp,
  * hamiltonianMatrix.h:1431-1970 (same as applyHconf_XX 891-1425): the same
  * rules, multipliers and order, out = w + 0.5*z.  The chunk ordinals are
  * hoisted out of the innermost loops (same values).
  * Pinned by: dense brute-force second quantization (tiny bases), Hermiticity,
  * agreement with the hash-only restatement, and the reference build. */
-void orc_acth(int N, int M, const int32_t *Map, const int32_t *MapOT,
-              const int32_t *MapTT, const int64_t *strideOT,
-              const int64_t *strideTT, const int32_t *IF, const cplx *C,
-              const cplx *Ho, const cplx *Hint, cplx *out)
+void orc_acth_range(int N, int M, int64_t row0, int64_t row1, const int32_t *Map,
+                    const int32_t *MapOT, const int32_t *MapTT, const int64_t *strideOT,
+                    const int64_t *strideTT, const int32_t *IF, const cplx *C,
+                    const cplx *Ho, const cplx *Hint, cplx *out_rows)
 {
     int64_t nc = orc_nc(N, M);
     int64_t M2 = (int64_t)M * M;
+    cplx *out = out_rows - row0;   /* out[i] for i in [row0, row1) */
 #pragma omp parallel
     {
         int32_t *v = (int32_t *)malloc(sizeof(int32_t) * (M > 0 ? M : 1));
 #pragma omp for schedule(static)
-        for (int64_t i = 0; i < nc; i++) {
+        for (int64_t i = row0; i < row1; i++) {
             cplx w = 0, z = 0;
             for (int k = 0; k < M; k++) v[k] = IF[i * M + k];
 
@@ -450,6 +452,16 @@ void orc_acth(int N, int M, const int32_t *Map, const int32_t *MapOT,
         }
         free(v);
     }
+}
+
+/* all rows: restates applyHconf_omp over the whole basis */
+void orc_acth(int N, int M, const int32_t *Map, const int32_t *MapOT,
+              const int32_t *MapTT, const int64_t *strideOT,
+              const int64_t *strideTT, const int32_t *IF, const cplx *C,
+              const cplx *Ho, const cplx *Hint, cplx *out)
+{
+    orc_acth_range(N, M, 0, orc_nc(N, M), Map, MapOT, MapTT, strideOT, strideTT, IF, C, Ho,
+                   Hint, out);
 }
 
 /* Action of H on C for a list of rows, ranking every jump on the fly (no

@@ -19,6 +19,17 @@ pytestmark = pytest.mark.gpu
 CASES = [(1, 1), (3, 1), (0, 3), (1, 3), (2, 3), (5, 5), (4, 6), (6, 4), (3, 7), (8, 5), (2, 9), (6, 9)]
 
 
+def schedules(M, maps):
+    """1 generic; 2 specialised (reads every stored map); 3 specialised,
+    double jumps ranked on the fly (differs from 2 only with full maps)."""
+    out = [1]
+    if 3 <= M <= 13:
+        out.append(2)
+        if maps == "full":
+            out.append(3)
+    return out
+
+
 @pytest.mark.parametrize("N,M", CASES)
 @pytest.mark.parametrize("maps", ["full", "single", "none"])
 def test_apply_random(oracle_lib, N, M, maps):
@@ -28,8 +39,34 @@ def test_apply_random(oracle_lib, N, M, maps):
     ref = o.acth(C, Ho, Hint)
     g = mcpi_amd.Basis(N, M, maps)
     for skip in (True, False):
-        H = mcpi_amd.Hamiltonian(g, Ho, Hint, skip_zero=skip)
-        assert_close(H(C), ref, 1e-12, f"{maps} skip={skip}")
+        for sched in schedules(M, maps):
+            H = mcpi_amd.Hamiltonian(g, Ho, Hint, skip_zero=skip, schedule=sched)
+            assert_close(H(C), ref, 1e-12, f"{maps} skip={skip} schedule={sched}")
+        H = mcpi_amd.Hamiltonian(g, Ho, Hint, skip_zero=skip)   # autotuned
+        assert_close(H(C), ref, 1e-12, f"{maps} skip={skip} auto")
+        assert H.schedule in schedules(M, maps)
+
+
+def test_apply_sparse_hint_all_schedules(oracle_lib):
+    """A Hint with scattered exact zeros and a non-diagonal Ho: the
+    zero-skipping lists must drop exactly the zero terms."""
+    N, M = 6, 7
+    o = oracle.Basis(N, M)
+    C, Ho, Hint = oracle.random_inputs(N, M, seed=99)
+    rng = np.random.default_rng(4)
+    mask = rng.random((M,) * 4) < 0.6
+    mask = mask & mask.transpose(1, 0, 2, 3) & mask.transpose(0, 1, 3, 2) & mask.transpose(2, 3, 0, 1)
+    Hint = np.where(mask, Hint, 0)
+    Ho = np.where(rng.random((M, M)) < 0.5, Ho, 0)
+    Ho = np.triu(Ho) + np.triu(Ho, 1).conj().T
+    ref = o.acth(C, Ho, Hint)
+    for maps in ("full", "single", "none"):
+        g = mcpi_amd.Basis(N, M, maps)
+        for sched in schedules(M, maps):
+            for skip in (True, False):
+                note(f"sparse hint maps={maps} schedule={sched} skip={skip}")
+                H = mcpi_amd.Hamiltonian(g, Ho, Hint, skip_zero=skip, schedule=sched)
+                assert_close(H(C), ref, 1e-12, f"{maps} {sched} {skip}")
 
 
 @pytest.mark.parametrize("N,M,g", [(5, 11, 10.0), (8, 9, 0.3), (6, 12, 50.0)])
@@ -42,7 +79,9 @@ def test_apply_lieb_liniger(oracle_lib, N, M, g, maps):
     ref = o.acth(C, Ho, Hint)
     b = mcpi_amd.Basis(N, M, maps)
     for skip in (True, False):
-        assert_close(mcpi_amd.Hamiltonian(b, Ho, Hint, skip_zero=skip)(C), ref, 1e-12, f"skip={skip}")
+        for sched in schedules(M, maps):
+            H = mcpi_amd.Hamiltonian(b, Ho, Hint, skip_zero=skip, schedule=sched)
+            assert_close(H(C), ref, 1e-12, f"skip={skip} schedule={sched}")
 
 
 GOLD = sorted(glob.glob(os.path.join(os.path.dirname(__file__), "golden", "*.npz")))
@@ -109,8 +148,10 @@ def test_n12_m12_lieb_liniger_sampled_rows(oracle_lib, maps):
     rows = np.concatenate([[0, 1, b.nc - 1], rng.integers(0, b.nc, 4000)])
     ref = oracle.acth_hash_rows(N, M, rows, C, Ho, Hint)
     assert_close(out[rows], ref, 1e-12, "rows")
-    H2 = mcpi_amd.Hamiltonian(b, Ho, Hint, skip_zero=False)
-    assert_close(H2(C), out, 1e-12, "skip off")
+    for sched in schedules(M, maps):
+        for skip in (False, True):
+            H2 = mcpi_amd.Hamiltonian(b, Ho, Hint, skip_zero=skip, schedule=sched)
+            assert_close(H2(C), out, 1e-12, f"schedule {sched} skip {skip}")
     _hermitian_check(H, b.nc, 7)
 
 
@@ -126,6 +167,9 @@ def test_n12_m12_generic_sampled_rows(oracle_lib):
     out = H(C)
     rows = np.concatenate([[0, b.nc - 1], rng.integers(0, b.nc, 1500)])
     assert_close(out[rows], oracle.acth_hash_rows(N, M, rows, C, Ho, Hint), 1e-12, "rows")
+    for sched in (1, 2, 3):
+        H2 = mcpi_amd.Hamiltonian(b, Ho, Hint, skip_zero=False, schedule=sched)
+        assert_close(H2(C), out, 1e-12, f"schedule {sched}")
 
 
 def test_n16_m12_single_map_sampled_rows(oracle_lib):
