@@ -241,11 +241,16 @@ mcpi_status mcpi_hip_ground(mcpi_ham h, int lm, const mcpi_cplx *C0_dev,
 mcpi_status mcpi_hip_tridiag_lowest(int n, const double *d, const double *e,
                                     double *lambda, double *z);
 
-/* Batched Lanczos: n instances, each a Hamiltonian handle (instances may
- * share a basis) and a device start vector; lm steps each; E0[n] out. */
+/* Batched Lanczos ground states (groundStateLieb.c:348-471 for n instances
+ * at once): n Hamiltonian handles (instances may share a basis and differ in
+ * Ho/Hint, N, M) and normalized device start vectors; lm steps each, all
+ * instances advanced together, one iteration replayed as one hipGraph.
+ * Out: E0[n], steps[n] (host); C_out_dev (may be NULL, or hold NULL
+ * entries): device Ritz vectors, nc each (may alias the start vectors). */
 mcpi_status mcpi_hip_lanczos_batched(int n, const mcpi_ham *hams,
                                      const mcpi_cplx *const *C0_dev, int lm,
-                                     double *E0, int *steps);
+                                     double *E0, int *steps,
+                                     mcpi_cplx *const *C_out_dev);
 
 #ifdef __cplusplus
 }

@@ -103,7 +103,7 @@ def lib():
                                  ct.POINTER(ct.c_int)]),
         "mcpi_hip_tridiag_lowest": (st, [ct.c_int, _f64p, _f64p, ct.POINTER(ct.c_double), _f64p]),
         "mcpi_hip_lanczos_batched": (st, [ct.c_int, ct.POINTER(_vp), ct.POINTER(_vp), ct.c_int,
-                                          _f64p, _i32p]),
+                                          _f64p, _i32p, ct.POINTER(_vp)]),
     }
     for name, (res, args) in sig.items():
         f = getattr(L, name)
@@ -376,15 +376,30 @@ def tridiag_lowest(d, e):
     return lam.value, z
 
 
-def lanczos_batched(hams, starts, lm: int):
-    """Lanczos ground states of many instances; returns (E0[n], steps[n])."""
+def lanczos_batched(hams, starts, lm: int, outs=None):
+    """Lanczos ground states of many instances advanced together
+    (mcpi_hip_lanczos_batched).  hams: Hamiltonian objects (instances may
+    share a Basis); starts: normalized DeviceArray start vectors; outs:
+    optional DeviceArrays (or None entries) for the Ritz vectors.  Returns
+    (E0[n], steps[n])."""
     n = len(hams)
-    H = (_vp * n)(*[h.handle for h in hams])
-    S = (_vp * n)(*[s.ptr for s in starts])
-    E0 = np.zeros(n, np.float64)
-    steps = np.zeros(n, np.int32)
-    _check(lib().mcpi_hip_lanczos_batched(n, H, S, lm, E0, steps), "lanczos_batched")
-    return E0, steps
+    if len(starts) != n or (outs is not None and len(outs) != n):
+        raise ValueError("one start vector (and output) per instance")
+    for h, s in zip(hams, starts):
+        if s.n != h.basis.nc:
+            raise ValueError("start vector length differs from the instance's basis size")
+    H = (_vp * max(n, 1))(*[h.handle for h in hams])
+    S = (_vp * max(n, 1))(*[s.ptr for s in starts])
+    O = None
+    if outs is not None:
+        for h, o in zip(hams, outs):
+            if o is not None and o.n != h.basis.nc:
+                raise ValueError("output length differs from the instance's basis size")
+        O = (_vp * max(n, 1))(*[(o.ptr if o is not None else None) for o in outs])
+    E0 = np.zeros(max(n, 1), np.float64)
+    steps = np.zeros(max(n, 1), np.int32)
+    _check(lib().mcpi_hip_lanczos_batched(n, H, S, lm, E0, steps, O), "lanczos_batched")
+    return E0[:n], steps[:n]
 
 
 # ----------------------------------------------------------- models --

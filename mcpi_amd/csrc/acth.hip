@@ -27,7 +27,8 @@ namespace mcpi {
 
 // --------------------------------------------------------- coefficients --
 
-mcpi_status build_coefficients(int M, const cplx *Ho, const cplx *Hint, HamCoefDev *c, void **block)
+mcpi_status build_coefficients(int M, const cplx *Ho, const cplx *Hint, HamCoefDev *c, void **block,
+                               size_t *block_bytes)
 {
     const int64_t M2 = (int64_t)M * M, M3 = M2 * M;
     const int P = M * (M - 1) / 2;
@@ -104,12 +105,16 @@ mcpi_status build_coefficients(int M, const cplx *Ho, const cplx *Hint, HamCoefD
             if (nzt[p * M2 + e]) { nz_e.push_back(e); nz_c.push_back(tt[p * M2 + e]); }
     }
     tt_off[P] = (int32_t)nz_e.size();
-    if (nz_e.empty()) { nz_e.push_back(0); nz_c.push_back(cmake(0, 0)); }
+    // the lists are padded to their largest possible length, so that every
+    // Hamiltonian with M orbitals has the same block layout and size (batched
+    // H.v addresses instance x's tables at a fixed stride from instance 0's)
+    nz_e.resize((size_t)(M + std::max(P, 1)) * M2 + 1, 0);
+    nz_c.resize(nz_e.size(), cmake(0, 0));
     // one device block: complex tables first (16-byte aligned), then integers
     size_t nC = dk.size() + d1.size() + d2.size() + A0.size() + A.size() + ot.size() + tt.size() + nz_c.size();
     size_t nB = nz1.size() + nzo.size() + nzt.size() + 4 * (m1j.size() + ot_off.size() + tt_off.size() +
                                                            nz_e.size()) + 64;
-    size_t bytes = nC * sizeof(cplx) + nB + 64;
+    size_t bytes = (nC * sizeof(cplx) + nB + 64 + 255) & ~(size_t)255;
     char *dev = nullptr;
     MCPI_HIP(hipMalloc(&dev, bytes));
     std::vector<char> host(bytes, 0);
@@ -155,6 +160,7 @@ mcpi_status build_coefficients(int M, const cplx *Ho, const cplx *Hint, HamCoefD
         for (const cplx &z : *v) real = real && z.im == 0.0;
     c->real = real ? 1 : 0;
     *block = dev;
+    if (block_bytes) *block_bytes = bytes;
     return MCPI_OK;
 }
 
@@ -315,19 +321,38 @@ static size_t rows_smem(int N, int M, int T)
     return sizeof(double) * (N + 3) + sizeof(int32_t) * M * (N + 1) + 2 * (size_t)M * T + 16;
 }
 
-mcpi_status launch_fast_a(const BasisDev &, int, const HamCoefDev &, int, int, const cplx *, cplx *, hipStream_t);
-mcpi_status launch_fast_b(const BasisDev &, int, const HamCoefDev &, int, int, const cplx *, cplx *, hipStream_t);
-mcpi_status launch_fast_c(const BasisDev &, int, const HamCoefDev &, int, int, const cplx *, cplx *, hipStream_t);
+#define MCPI_FAST_ARGS const BasisDev &, int, const HamCoefDev &, int, int, const cplx *, cplx *, hipStream_t, \
+                       int, size_t, int64_t
+mcpi_status launch_fast_a(MCPI_FAST_ARGS);
+mcpi_status launch_fast_b(MCPI_FAST_ARGS);
+mcpi_status launch_fast_c(MCPI_FAST_ARGS);
+#undef MCPI_FAST_ARGS
 
-// specialised kernel for b.M, or MCPI_ERR_STATE when there is none
+// specialised kernel for b.M, or MCPI_ERR_STATE when there is none.
+// nbatch > 0: one launch over nbatch instances laid out at fixed strides
+// (coefficient tables cstride bytes apart, vectors vstride elements apart)
 static mcpi_status launch_fast(const BasisDev &b, int maps, const HamCoefDev &c, int skip, int rank_doubles,
-                               const cplx *C, cplx *out, hipStream_t st)
+                               const cplx *C, cplx *out, hipStream_t st, int nbatch = 0, size_t cstride = 0,
+                               int64_t vstride = 0)
 {
     if (b.N > MCPI_MAXN) return MCPI_ERR_STATE;
-    if (b.M >= 3 && b.M <= 6) return launch_fast_a(b, maps, c, skip, rank_doubles, C, out, st);
-    if (b.M >= 7 && b.M <= 10) return launch_fast_b(b, maps, c, skip, rank_doubles, C, out, st);
-    if (b.M >= 11 && b.M <= 13) return launch_fast_c(b, maps, c, skip, rank_doubles, C, out, st);
+    if (b.M >= 3 && b.M <= 6) return launch_fast_a(b, maps, c, skip, rank_doubles, C, out, st, nbatch, cstride, vstride);
+    if (b.M >= 7 && b.M <= 10)
+        return launch_fast_b(b, maps, c, skip, rank_doubles, C, out, st, nbatch, cstride, vstride);
+    if (b.M >= 11 && b.M <= 13)
+        return launch_fast_c(b, maps, c, skip, rank_doubles, C, out, st, nbatch, cstride, vstride);
     return MCPI_ERR_STATE;
+}
+
+// One launch of the specialised schedule `sched` (2 or 3) over nbatch
+// instances sharing a basis.  c0: instance 0's tables; instance x's are c0's
+// shifted by x * cstride bytes (same layout, build_coefficients), its input
+// and output vectors C0 + x * vstride, out0 + x * vstride.  c0.real must hold
+// for every instance.  MCPI_ERR_STATE if no specialisation exists.
+mcpi_status launch_apply_batch(const BasisDev &b, int maps, const HamCoefDev &c0, int skip, int sched, int nbatch,
+                               size_t cstride, const cplx *C0, cplx *out0, int64_t vstride, hipStream_t st)
+{
+    return launch_fast(b, maps, c0, skip, sched == 3, C0, out0, st, nbatch, cstride, vstride);
 }
 
 bool has_fast_kernel(int N, int M) { return N <= MCPI_MAXN && M >= 3 && M <= 13; }

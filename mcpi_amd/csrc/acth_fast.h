@@ -240,10 +240,25 @@ __device__ inline cplx chunk_list(const int32_t *__restrict__ chunk, const uint3
 // Occupancy: the zero-skipping kernels fit in 128 VGPRs (4 waves per SIMD);
 // the every-term kernels keep a whole chunk's gathers in flight and are
 // given up to 256 VGPRs (2 waves per SIMD) rather than spill.
-template <int M, bool MAP1, bool DMAP, bool SKIP, bool RC>
+// BATCH: nbatch instances share the basis.  Instance x's coefficient tables
+// are instance 0's shifted by x * cstride bytes and its vectors sit
+// x * vstride elements after C0 / out0, so every pointer derives from a
+// kernel argument and keeps its global / scalar-load form.  The 1-D grid
+// holds nbatch blocks per row block with the instance index fastest, after
+// the XCD remap: the nbatch blocks of one row range run back to back on the
+// same XCD, so the row range's tables are read from HBM once and served to
+// the other instances from that XCD's L2.
+template <int M, bool MAP1, bool DMAP, bool SKIP, bool RC, bool BATCH>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(SKIP ? 4 : 2))) void k_acth_fast(
-    BasisDev b, HamCoefDev c, const cplx *__restrict__ C, cplx *__restrict__ out)
+    BasisDev b, HamCoefDev c0, const cplx *__restrict__ C0, cplx *__restrict__ out0, int nbatch, size_t cstride,
+    int64_t vstride)
 {
+    const unsigned vb = xcd_block(blockIdx.x, gridDim.x);
+    const unsigned inst = BATCH ? vb % (unsigned)nbatch : 0u;
+    const unsigned rb = BATCH ? vb / (unsigned)nbatch : vb;
+    const HamCoefDev c = BATCH ? shift_coef(c0, (ptrdiff_t)((size_t)inst * cstride)) : c0;
+    const cplx *__restrict__ C = BATCH ? C0 + (int64_t)inst * vstride : C0;
+    cplx *__restrict__ out = BATCH ? out0 + (int64_t)inst * vstride : out0;
     constexpr int MM = M * M;
     extern __shared__ double fsm[];
     double *sq = fsm;                                   // N+3
@@ -254,7 +269,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(SKIP ? 4 : 
     if (!MAP1 || !DMAP)
         for (int x = threadIdx.x; x < M * (b.N + 1); x += blockDim.x) Bt[x] = (uint32_t)b.B[x];
     __syncthreads();
-    const int64_t i = (int64_t)xcd_block(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
+    const int64_t i = (int64_t)rb * blockDim.x + threadIdx.x;
     if (i >= b.nc) return;
     const int N = b.N;
     const int64_t nc = b.nc;
@@ -386,19 +401,29 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(SKIP ? 4 : 
 }
 
 // launch one specialisation; rank_doubles: rank double jumps even when the
-// maps are stored; real: every folded coefficient has a zero imaginary part
+// maps are stored; real: every folded coefficient has a zero imaginary part.
+// nbatch > 0: batched launch over nbatch instances at fixed strides.
 template <int M>
 inline mcpi_status launch_fast_m(const BasisDev &b, int maps, const HamCoefDev &c, int skip, int rank_doubles,
-                                 const cplx *C, cplx *out, hipStream_t st)
+                                 const cplx *C, cplx *out, hipStream_t st, int nbatch = 0, size_t cstride = 0,
+                                 int64_t vstride = 0)
 {
-    dim3 grid((unsigned)((b.nc + 255) / 256)), blk(256);
+    dim3 grid((unsigned)((b.nc + 255) / 256) * (unsigned)(nbatch > 0 ? nbatch : 1)), blk(256);
     size_t sm = sizeof(double) * (b.N + 3) + sizeof(uint32_t) * M * (b.N + 1) + 16;
     if (skip) sm += sizeof(uint32_t) * 2 * M * 256;
     const bool map1 = maps >= MCPI_MAPS_SINGLE;
     const bool dmap = maps == MCPI_MAPS_FULL && !rank_doubles;
     const bool real = c.real;
-#define MCPI_L(A, B, S, R) hipLaunchKernelGGL((k_acth_fast<M, A, B, S, R>), grid, blk, sm, st, b, c, C, out)
-#define MCPI_L2(A, B, S) { if (real) MCPI_L(A, B, S, true); else MCPI_L(A, B, S, false); }
+#define MCPI_L(A, B, S, R)                                                                                  \
+    {                                                                                                       \
+        if (nbatch > 0)                                                                                     \
+            hipLaunchKernelGGL((k_acth_fast<M, A, B, S, R, true>), grid, blk, sm, st, b, c, C, out, nbatch, \
+                               cstride, vstride);                                                                       \
+        else                                                                                                \
+            hipLaunchKernelGGL((k_acth_fast<M, A, B, S, R, false>), grid, blk, sm, st, b, c, C, out, nbatch, \
+                               cstride, vstride);                                                                       \
+    }
+#define MCPI_L2(A, B, S) { if (real) MCPI_L(A, B, S, true) else MCPI_L(A, B, S, false) }
     if (map1 && dmap) { if (skip) MCPI_L2(true, true, true) else MCPI_L2(true, true, false) }
     else if (map1) { if (skip) MCPI_L2(true, false, true) else MCPI_L2(true, false, false) }
     else { if (skip) MCPI_L2(false, false, true) else MCPI_L2(false, false, false) }

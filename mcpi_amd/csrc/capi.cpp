@@ -34,6 +34,8 @@ mcpi_status ground_run(mcpi_ham h, int lm, const cplx *C0, cplx *Cout, double *E
 mcpi_status lanczos_api(mcpi_ham h, int lm, const cplx *C0, cplx *lvec, cplx *diag, cplx *offd, int *steps,
                         hipStream_t s);
 mcpi_status tridiag_lowest(int n, const double *d, const double *e, double *lambda, double *z);
+mcpi_status lanczos_batched_api(int K, const mcpi_ham *hams, const cplx *const *C0, int lm, double *E0,
+                                int *steps, cplx *const *Cout, hipStream_t s);
 
 // one library stream (default priority) for calls that do not pass one
 static hipStream_t lib_stream()
@@ -110,7 +112,8 @@ mcpi_status mcpi_hip_ham_create(mcpi_basis b, const mcpi_cplx *Ho, const mcpi_cp
     h->schedule = 0;
     h->tuned[0] = h->tuned[1] = 0;
     h->Ho_dev = h->Hint_dev = nullptr;
-    mcpi_status st = build_coefficients(M, (const cplx *)Ho, (const cplx *)Hint, &h->c, &h->coef_block);
+    mcpi_status st = build_coefficients(M, (const cplx *)Ho, (const cplx *)Hint, &h->c, &h->coef_block,
+                                        &h->coef_bytes);
     if (st == MCPI_OK) {
         const size_t nHo = (size_t)M * M, nHi = nHo * nHo;
         hipError_t e = hipMalloc(&h->Ho_dev, sizeof(cplx) * nHo);
@@ -245,21 +248,10 @@ mcpi_status mcpi_hip_tridiag_lowest(int n, const double *d, const double *e, dou
 }
 
 mcpi_status mcpi_hip_lanczos_batched(int n, const mcpi_ham *hams, const mcpi_cplx *const *C0_dev, int lm,
-                                     double *E0, int *steps)
+                                     double *E0, int *steps, mcpi_cplx *const *C_out_dev)
 {
-    MCPI_ARG(n >= 0 && hams && C0_dev && E0 && lm >= 1, "bad argument");
-    for (int x = 0; x < n; x++) {
-        MCPI_ARG(hams[x] && C0_dev[x], "null instance");
-        const int64_t nc = hams[x]->basis->d.nc;
-        cplx *out = nullptr;
-        MCPI_HIP(hipMalloc(&out, sizeof(cplx) * nc));
-        int st = 0;
-        mcpi_status s = ground_run(hams[x], lm, (const cplx *)C0_dev[x], out, &E0[x], &st, lib_stream(), nullptr);
-        (void)hipFree(out);
-        MCPI_CHECK(s);
-        if (steps) steps[x] = st;
-    }
-    return MCPI_OK;
+    return lanczos_batched_api(n, hams, (const cplx *const *)C0_dev, lm, E0, steps, (cplx *const *)C_out_dev,
+                               lib_stream());
 }
 
 }  // extern "C"

@@ -1,25 +1,32 @@
-// Lanczos ground state on the device.
+// Lanczos ground states on the device, one instance or many in lockstep.
 //
 // Reference: lanczos, groundStateLieb.c:217-344 (three-term recurrence with
 // one pass of classical Gram-Schmidt against every previous vector),
 // carrDot / carrMod (73-101) and ground (348-471: tridiagonal eigenproblem on
-// the host, Ritz vector back-transform).
+// the host, Ritz vector back-transform 454-458).
 //
-// One iteration i (0 <= i < lm-1) is a fixed chain of kernels that read the
-// step index and the scalars from device memory:
+// K instances run together.  Each has its own Hamiltonian, vectors and
+// scalars (LzInst descriptor, LanczosState); instances that share a basis may
+// differ in Ho/Hint.  Every kernel below serves all K instances in one launch
+// (instance = blockIdx.y, blockIdx.z for the Gram-Schmidt dots), so one
+// iteration is the same short chain of launches for any K:
 //   1 fin_beta      beta_i = ||HC|| from block partials, breakdown test
 //   2 scale         v_{i+1} = HC / beta_i  (into lvec[i+1] and the H.v input)
-//   3 H.v           HC = H v_{i+1}
+//   3 H.v           HC = H v_{i+1}: one launch per group of instances that
+//                   share a basis (the group's row blocks are interleaved so
+//                   its instances read each table row from one XCD's L2)
 //   4 sub_dot       HC -= beta_i v_i;  partial <v_{i+1}, HC>
-//   5 fin_scalar    alpha_{i+1}
+//   5 fin_alpha     alpha_{i+1}
 //   6 sub_alpha     HC -= alpha v_{i+1}
 //     gemv_dots     partial o_t = <v_t, HC>, t <= i+1
 //   7 fin_ortho     o_t
 //   8 reorth_norm   HC -= sum_t v_t o_t;  partial ||HC||^2 for the next step
 //   9 advance       i += 1
-// The pointers never change between iterations, so the chain is captured
-// once in a hipGraph and replayed lm-1 times (no host round trip per step).
-// Reductions are two-stage (block partials, then one block), deterministic.
+// Every scalar lives in device memory and the pointers never change, so the
+// chain is captured once in a hipGraph and replayed lm-1 times with no host
+// round trip.  An instance that breaks down stops updating (its kernels
+// return at once) while the others go on.  Reductions are two-stage (block
+// partials, then one block per instance) and deterministic.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -42,6 +49,22 @@ struct LanczosState {
     cplx alpha;    // alpha of the current iteration
 };
 
+// one instance: sizes and device buffers
+struct LzInst {
+    int64_t n;            // basis size nc
+    const cplx *C0;       // start vector
+    cplx *lvec;           // lm * n, row t = v_t
+    cplx *HC, *vin;       // H.v output and input
+    cplx *diag, *offd;    // lm each (real parts used)
+    cplx *ortho;          // lm Gram-Schmidt coefficients
+    double *partial;      // 2 * nb block partials
+    double *partial_o;    // 2 * nbg * lm dot partials
+    const double *z;      // lm Ritz coefficients (after the host solve)
+    cplx *Cout;           // Ritz vector (may be null)
+    int nb, nbg;          // vector and dot blocks used by this instance
+    int steps;            // Ritz length (after the host solve)
+};
+
 #define LZ_T 256
 
 __device__ inline double lz_wave_sum(double x)
@@ -50,7 +73,7 @@ __device__ inline double lz_wave_sum(double x)
     return x;
 }
 
-// block sum of (a, b) into partial[blockIdx] (2 doubles)
+// block sum of (a, b) into partial2[2 * blockIdx.x .. +1]
 __device__ inline void block_sum2(double a, double b, double *partial2)
 {
     __shared__ double red[2][LZ_T / 64];
@@ -67,54 +90,104 @@ __device__ inline void block_sum2(double a, double b, double *partial2)
     }
 }
 
-__device__ inline cplx ldv(const cplx *p)
+// The descriptors hold generic pointers; vector accesses go through global
+// (address space 1) pointers and the per-step coefficients through constant
+// (address space 4) ones, so that the compiler emits global and scalar loads
+// instead of flat ones.
+#define LZ_GLOBAL __attribute__((address_space(1)))
+#define LZ_CONST __attribute__((address_space(4)))
+template <class T>
+__device__ inline LZ_GLOBAL T *gp(T *p)
 {
-    double2 t = *reinterpret_cast<const double2 *>(p);
+    return (LZ_GLOBAL T *)p;
+}
+template <class T>
+__device__ inline const LZ_CONST T *kp(const T *p)
+{
+    return (const LZ_CONST T *)p;
+}
+
+typedef double lz_d2 __attribute__((ext_vector_type(2)));   // one 16-byte access
+
+__device__ inline cplx ldv(const LZ_GLOBAL cplx *p)
+{
+    lz_d2 t = *(const LZ_GLOBAL lz_d2 *)p;
     return cmake(t.x, t.y);
 }
-__device__ inline void stv(cplx *p, cplx z)
+__device__ inline void stv(LZ_GLOBAL cplx *p, cplx z)
 {
-    double2 t;
+    lz_d2 t;
     t.x = z.re;
     t.y = z.im;
-    *reinterpret_cast<double2 *>(p) = t;
+    *(LZ_GLOBAL lz_d2 *)p = t;
 }
+
+// grid-stride loop over this instance's rows; blocks beyond its nb return
+// before any barrier (the test is uniform over the block)
+#define LZ_ROWS(I, j) \
+    for (int64_t j = (int64_t)blockIdx.x * LZ_T + threadIdx.x; j < (I).n; j += (int64_t)(I).nb * LZ_T)
 
 // ---- set-up kernels (before the first iteration) --
 
-__global__ void lz_init(LanczosState *st, int lm, double tol)
+// state, and diag/offdiag zeroed (grid (1, K))
+__global__ void lz_init(LanczosState *sts, const LzInst *in, int lm, double tol)
 {
-    st->it = 0;
-    st->broke = 0;
-    st->steps = lm;
-    st->lm = lm;
-    st->maxb = 0;
-    st->tol = tol;
-    st->beta = 0;
-    st->alpha = cmake(0, 0);
+    LanczosState *st = sts + blockIdx.y;
+    const LzInst &I = in[blockIdx.y];
+    for (int x = threadIdx.x; x < lm; x += blockDim.x) {
+        I.diag[x] = cmake(0, 0);
+        I.offd[x] = cmake(0, 0);
+    }
+    if (threadIdx.x == 0) {
+        st->it = 0;
+        st->broke = 0;
+        st->steps = lm;
+        st->lm = lm;
+        st->maxb = 0;
+        st->tol = tol;
+        st->beta = 0;
+        st->alpha = cmake(0, 0);
+    }
 }
 
-// partial <x, y> (x conjugated)
-__global__ __launch_bounds__(LZ_T) void lz_dot(int64_t n, const cplx *__restrict__ x,
-                                              const cplx *__restrict__ y, double *partial)
+// v_0 = C0 into lvec[0] and the H.v input (C0 may alias lvec)
+__global__ __launch_bounds__(LZ_T) void lz_start(const LzInst *in)
 {
+    const LzInst &I = in[blockIdx.y];
+    if ((int)blockIdx.x >= I.nb) return;
+    LZ_ROWS(I, j)
+    {
+        cplx x = ldv(gp(I.C0) + j);
+        stv(gp(I.lvec) + j, x);
+        stv(gp(I.vin) + j, x);
+    }
+}
+
+// partial <vin, HC>
+__global__ __launch_bounds__(LZ_T) void lz_dot(const LzInst *in)
+{
+    const LzInst &I = in[blockIdx.y];
+    if ((int)blockIdx.x >= I.nb) return;
     double re = 0, im = 0;
-    for (int64_t j = (int64_t)blockIdx.x * LZ_T + threadIdx.x; j < n; j += (int64_t)gridDim.x * LZ_T) {
-        cplx a = ldv(x + j), b = ldv(y + j);
+    LZ_ROWS(I, j)
+    {
+        cplx a = ldv(gp(I.vin) + j), b = ldv(gp(I.HC) + j);
         re += a.re * b.re + a.im * b.im;
         im += a.re * b.im - a.im * b.re;
     }
-    block_sum2(re, im, partial);
+    block_sum2(re, im, I.partial);
 }
 
-// sum block partials into a complex scalar; which: 0 -> alpha (diag[it+1] or
-// diag[0] when first), 1 -> first alpha into diag[0]
-__global__ void lz_fin_alpha(LanczosState *st, int nb, const double *partial, cplx *diag, int first)
+// alpha from the block partials, into diag[0] (first) or diag[it + 1]
+// (grid (1, K))
+__global__ void lz_fin_alpha(LanczosState *sts, const LzInst *in, int first)
 {
     __shared__ double red[2][LZ_T];
+    LanczosState *st = sts + blockIdx.y;
+    const LzInst &I = in[blockIdx.y];
     if (st->broke) return;
     double re = 0, im = 0;
-    for (int x = threadIdx.x; x < nb; x += LZ_T) { re += partial[2 * x]; im += partial[2 * x + 1]; }
+    for (int x = threadIdx.x; x < I.nb; x += LZ_T) { re += I.partial[2 * x]; im += I.partial[2 * x + 1]; }
     red[0][threadIdx.x] = re;
     red[1][threadIdx.x] = im;
     __syncthreads();
@@ -122,36 +195,40 @@ __global__ void lz_fin_alpha(LanczosState *st, int nb, const double *partial, cp
         double a = 0, b = 0;
         for (int x = 0; x < LZ_T; x++) { a += red[0][x]; b += red[1][x]; }
         st->alpha = cmake(a, b);
-        diag[first ? 0 : st->it + 1] = cmake(a, b);
+        I.diag[first ? 0 : st->it + 1] = cmake(a, b);
     }
 }
 
-// HC -= alpha v;  partial ||HC||^2   (set-up only: iteration 0 has no reorthogonalization)
-__global__ __launch_bounds__(LZ_T) void lz_sub_alpha_norm(int64_t n, const LanczosState *st,
-                                                        const cplx *__restrict__ v, cplx *HC,
-                                                        double *partial)
+// HC -= alpha v;  partial ||HC||^2   (set-up only: iteration 0 has no
+// reorthogonalization)
+__global__ __launch_bounds__(LZ_T) void lz_sub_alpha_norm(const LanczosState *sts, const LzInst *in)
 {
-    const cplx a = st->alpha;
+    const LzInst &I = in[blockIdx.y];
+    if ((int)blockIdx.x >= I.nb) return;
+    const cplx a = sts[blockIdx.y].alpha;
     double s = 0;
-    for (int64_t j = (int64_t)blockIdx.x * LZ_T + threadIdx.x; j < n; j += (int64_t)gridDim.x * LZ_T) {
-        cplx h = ldv(HC + j), x = ldv(v + j);
+    LZ_ROWS(I, j)
+    {
+        cplx h = ldv(gp(I.HC) + j), x = ldv(gp(I.vin) + j);
         h.re -= a.re * x.re - a.im * x.im;
         h.im -= a.re * x.im + a.im * x.re;
-        stv(HC + j, h);
+        stv(gp(I.HC) + j, h);
         s += h.re * h.re + h.im * h.im;
     }
-    block_sum2(s, 0, partial);
+    block_sum2(s, 0, I.partial);
 }
 
 // ---- iteration kernels --
 
-// 1: beta_i and the breakdown test (groundStateLieb.c:287-292)
-__global__ void lz_fin_beta(LanczosState *st, int nb, const double *partial, cplx *offdiag)
+// 1: beta_i and the breakdown test (groundStateLieb.c:287-292); grid (1, K)
+__global__ void lz_fin_beta(LanczosState *sts, const LzInst *in)
 {
     __shared__ double red[LZ_T];
+    LanczosState *st = sts + blockIdx.y;
+    const LzInst &I = in[blockIdx.y];
     if (st->broke) return;
     double s = 0;
-    for (int x = threadIdx.x; x < nb; x += LZ_T) s += partial[2 * x];
+    for (int x = threadIdx.x; x < I.nb; x += LZ_T) s += I.partial[2 * x];
     red[threadIdx.x] = s;
     __syncthreads();
     if (threadIdx.x == 0) {
@@ -159,7 +236,7 @@ __global__ void lz_fin_beta(LanczosState *st, int nb, const double *partial, cpl
         for (int x = 0; x < LZ_T; x++) t += red[x];
         double b = sqrt(t);
         const int i = st->it;
-        offdiag[i] = cmake(b, 0);
+        I.offd[i] = cmake(b, 0);
         if (b > st->maxb) st->maxb = b;
         st->beta = b;
         if (b == 0.0 || b / st->maxb < st->tol) {
@@ -171,80 +248,88 @@ __global__ void lz_fin_beta(LanczosState *st, int nb, const double *partial, cpl
 }
 
 // 2: v_{i+1} = HC / beta_i, stored in lvec[i+1] and in the H.v input buffer
-__global__ __launch_bounds__(LZ_T) void lz_scale(int64_t n, const LanczosState *st, const cplx *__restrict__ HC,
-                                               cplx *lvec, cplx *vin)
+__global__ __launch_bounds__(LZ_T) void lz_scale(const LanczosState *sts, const LzInst *in)
 {
-    if (st->broke) return;
+    const LanczosState *st = sts + blockIdx.y;
+    const LzInst &I = in[blockIdx.y];
+    if ((int)blockIdx.x >= I.nb || st->broke) return;
     const double inv = 1.0 / st->beta;
-    cplx *dst = lvec + (int64_t)(st->it + 1) * n;
-    for (int64_t j = (int64_t)blockIdx.x * LZ_T + threadIdx.x; j < n; j += (int64_t)gridDim.x * LZ_T) {
-        cplx h = ldv(HC + j);
+    LZ_GLOBAL cplx *dst = gp(I.lvec) + (int64_t)(st->it + 1) * I.n;
+    LZ_ROWS(I, j)
+    {
+        cplx h = ldv(gp(I.HC) + j);
         h.re *= inv;
         h.im *= inv;
         stv(dst + j, h);
-        stv(vin + j, h);
+        stv(gp(I.vin) + j, h);
     }
 }
 
 // 4: HC -= beta_i v_i;  partial <v_{i+1}, HC>
-__global__ __launch_bounds__(LZ_T) void lz_sub_dot(int64_t n, const LanczosState *st, const cplx *lvec,
-                                                 const cplx *__restrict__ vin, cplx *HC, double *partial)
+__global__ __launch_bounds__(LZ_T) void lz_sub_dot(const LanczosState *sts, const LzInst *in)
 {
-    if (st->broke) return;
+    const LanczosState *st = sts + blockIdx.y;
+    const LzInst &I = in[blockIdx.y];
+    if ((int)blockIdx.x >= I.nb || st->broke) return;
     const double b = st->beta;
-    const cplx *vi = lvec + (int64_t)st->it * n;
+    const LZ_GLOBAL cplx *vi = gp(I.lvec) + (int64_t)st->it * I.n;
     double re = 0, im = 0;
-    for (int64_t j = (int64_t)blockIdx.x * LZ_T + threadIdx.x; j < n; j += (int64_t)gridDim.x * LZ_T) {
-        cplx h = ldv(HC + j), x = ldv(vi + j), y = ldv(vin + j);
+    LZ_ROWS(I, j)
+    {
+        cplx h = ldv(gp(I.HC) + j), x = ldv(vi + j), y = ldv(gp(I.vin) + j);
         h.re -= b * x.re;
         h.im -= b * x.im;
-        stv(HC + j, h);
+        stv(gp(I.HC) + j, h);
         re += y.re * h.re + y.im * h.im;
         im += y.re * h.im - y.im * h.re;
     }
-    block_sum2(re, im, partial);
+    block_sum2(re, im, I.partial);
 }
 
 // 6a: HC -= alpha v_{i+1}
-__global__ __launch_bounds__(LZ_T) void lz_sub_alpha(int64_t n, const LanczosState *st,
-                                                   const cplx *__restrict__ vin, cplx *HC)
+__global__ __launch_bounds__(LZ_T) void lz_sub_alpha(const LanczosState *sts, const LzInst *in)
 {
-    if (st->broke) return;
+    const LanczosState *st = sts + blockIdx.y;
+    const LzInst &I = in[blockIdx.y];
+    if ((int)blockIdx.x >= I.nb || st->broke) return;
     const cplx a = st->alpha;
-    for (int64_t j = (int64_t)blockIdx.x * LZ_T + threadIdx.x; j < n; j += (int64_t)gridDim.x * LZ_T) {
-        cplx x = ldv(HC + j), y = ldv(vin + j);
+    LZ_ROWS(I, j)
+    {
+        cplx x = ldv(gp(I.HC) + j), y = ldv(gp(I.vin) + j);
         x.re -= a.re * y.re - a.im * y.im;
         x.im -= a.re * y.im + a.im * y.re;
-        stv(HC + j, x);
+        stv(gp(I.HC) + j, x);
     }
 }
 
-// 6b: partial o_t = <v_t, HC> for t = 0..i+1.  Block (s, g) owns row
-// segment s (LZ_R rows per thread, kept in registers) and basis vectors
+// 6b: partial o_t = <v_t, HC> for t = 0..i+1.  Block (s, g, instance) owns
+// row segment s (LZ_R rows per thread, kept in registers) and basis vectors
 // t in [g*LZ_TG, (g+1)*LZ_TG); waves reduce with shuffles and park their sums
 // in LDS, one barrier per block.
 #define LZ_R 8
 #define LZ_TG 32
 #define LZ_SEG ((int64_t)LZ_T * LZ_R)
-__global__ __launch_bounds__(LZ_T) void lz_gemv_dots(int64_t n, const LanczosState *st, const cplx *lvec,
-                                                   const cplx *__restrict__ HC, double *partial_o, int lm)
+__global__ __launch_bounds__(LZ_T) void lz_gemv_dots(const LanczosState *sts, const LzInst *in, int lm)
 {
     __shared__ double red[LZ_TG][LZ_T / 64][2];
-    if (st->broke) return;
+    const LanczosState *st = sts + blockIdx.z;
+    const LzInst &I = in[blockIdx.z];
+    if ((int)blockIdx.x >= I.nbg || st->broke) return;
     const int nt = st->it + 2;
     const int t0 = blockIdx.y * LZ_TG;
     if (t0 >= nt) return;
     const int t1 = min(nt, t0 + LZ_TG);
+    const int64_t n = I.n;
     const int64_t base = (int64_t)blockIdx.x * LZ_SEG + threadIdx.x;
     cplx h[LZ_R];
 #pragma unroll
     for (int r = 0; r < LZ_R; r++) {
         int64_t j = base + (int64_t)r * LZ_T;
-        h[r] = (j < n) ? ldv(HC + j) : cmake(0, 0);
+        h[r] = (j < n) ? ldv(gp(I.HC) + j) : cmake(0, 0);
     }
     const int w = threadIdx.x >> 6;
     for (int tt = t0; tt < t1; tt++) {
-        const cplx *vt = lvec + (int64_t)tt * n;
+        const LZ_GLOBAL cplx *vt = gp(I.lvec) + (int64_t)tt * n;
         double re = 0, im = 0;
 #pragma unroll
         for (int r = 0; r < LZ_R; r++) {
@@ -268,183 +353,80 @@ __global__ __launch_bounds__(LZ_T) void lz_gemv_dots(int64_t n, const LanczosSta
         double sa = 0, sb = 0;
         for (int x = 0; x < LZ_T / 64; x++) { sa += red[tl][x][0]; sb += red[tl][x][1]; }
         int64_t at = 2 * ((int64_t)blockIdx.x * lm + t0 + tl);
-        partial_o[at] = sa;
-        partial_o[at + 1] = sb;
+        I.partial_o[at] = sa;
+        I.partial_o[at + 1] = sb;
     }
 }
 
-// 7: o_t = sum over blocks (one thread per t)
-__global__ void lz_fin_ortho(const LanczosState *st, int nb, const double *partial_o, int lm, cplx *ortho)
+// 7: o_t = sum over blocks (one thread per t); grid (1, K)
+__global__ void lz_fin_ortho(const LanczosState *sts, const LzInst *in, int lm)
 {
+    const LanczosState *st = sts + blockIdx.y;
+    const LzInst &I = in[blockIdx.y];
     if (st->broke) return;
     const int nt = st->it + 2;
     for (int tt = threadIdx.x; tt < nt; tt += blockDim.x) {
         double re = 0, im = 0;
-        for (int x = 0; x < nb; x++) {
-            re += partial_o[2 * ((int64_t)x * lm + tt)];
-            im += partial_o[2 * ((int64_t)x * lm + tt) + 1];
+        for (int x = 0; x < I.nbg; x++) {
+            re += I.partial_o[2 * ((int64_t)x * lm + tt)];
+            im += I.partial_o[2 * ((int64_t)x * lm + tt) + 1];
         }
-        ortho[tt] = cmake(re, im);
+        I.ortho[tt] = cmake(re, im);
     }
 }
 
-// 8: HC -= sum_t v_t o_t;  partial ||HC||^2 for the next beta
-__global__ __launch_bounds__(LZ_T) void lz_reorth_norm(int64_t n, const LanczosState *st, const cplx *lvec,
-                                                     const cplx *__restrict__ ortho, cplx *HC,
-                                                     double *partial)
+// 8: HC -= sum_t v_t o_t;  partial ||HC||^2 for the next beta.  The o_t are
+// the same for every lane: they are read as uniform (scalar-cache) loads.
+__global__ __launch_bounds__(LZ_T) void lz_reorth_norm(const LanczosState *sts, const LzInst *in)
 {
-    extern __shared__ double so[];
-    if (st->broke) return;
+    const LanczosState *st = sts + blockIdx.y;
+    const LzInst &I = in[blockIdx.y];
+    if ((int)blockIdx.x >= I.nb || st->broke) return;
     const int nt = st->it + 2;
-    for (int x = threadIdx.x; x < nt; x += LZ_T) {
-        cplx o = ortho[x];
-        so[2 * x] = o.re;
-        so[2 * x + 1] = o.im;
-    }
-    __syncthreads();
+    const int64_t n = I.n;
+    const LZ_CONST cplx *o = kp(I.ortho);
     double s = 0;
-    for (int64_t j = (int64_t)blockIdx.x * LZ_T + threadIdx.x; j < n; j += (int64_t)gridDim.x * LZ_T) {
-        cplx h = ldv(HC + j);
+    LZ_ROWS(I, j)
+    {
+        cplx h = ldv(gp(I.HC) + j);
 #pragma unroll 4
         for (int tt = 0; tt < nt; tt++) {
-            cplx x = ldv(lvec + (int64_t)tt * n + j);
-            double ore = so[2 * tt], oim = so[2 * tt + 1];
-            h.re -= x.re * ore - x.im * oim;
-            h.im -= x.re * oim + x.im * ore;
+            cplx x = ldv(gp(I.lvec) + (int64_t)tt * n + j);
+            const cplx ot = cmake(o[tt].re, o[tt].im);
+            h.re -= x.re * ot.re - x.im * ot.im;
+            h.im -= x.re * ot.im + x.im * ot.re;
         }
-        stv(HC + j, h);
+        stv(gp(I.HC) + j, h);
         s += h.re * h.re + h.im * h.im;
     }
-    block_sum2(s, 0, partial);
+    block_sum2(s, 0, I.partial);
 }
 
-// 9
-__global__ void lz_advance(LanczosState *st)
+// 9: one thread per instance
+__global__ void lz_advance(LanczosState *sts, int K)
 {
-    if (!st->broke) st->it += 1;
+    const int x = blockIdx.x * blockDim.x + threadIdx.x;
+    if (x < K && !sts[x].broke) sts[x].it += 1;
 }
 
-// Ritz vector C = sum_k lvec[k] z[k]  (groundStateLieb.c:454-458)
-__global__ __launch_bounds__(LZ_T) void lz_ritz(int64_t n, int steps, const cplx *lvec, const double *z, cplx *C)
+// Ritz vector C = sum_k lvec[k] z[k]  (groundStateLieb.c:454-458); z read
+// as uniform loads
+__global__ __launch_bounds__(LZ_T) void lz_ritz(const LzInst *in)
 {
-    extern __shared__ double sz[];
-    for (int x = threadIdx.x; x < steps; x += LZ_T) sz[x] = z[x];
-    __syncthreads();
-    for (int64_t j = (int64_t)blockIdx.x * LZ_T + threadIdx.x; j < n; j += (int64_t)gridDim.x * LZ_T) {
+    const LzInst &I = in[blockIdx.y];
+    if ((int)blockIdx.x >= I.nb || !I.Cout) return;
+    const int steps = I.steps;
+    const LZ_CONST double *z = kp(I.z);
+    LZ_ROWS(I, j)
+    {
         double re = 0, im = 0;
         for (int k = 0; k < steps; k++) {
-            cplx x = ldv(lvec + (int64_t)k * n + j);
-            re += x.re * sz[k];
-            im += x.im * sz[k];
+            cplx x = ldv(gp(I.lvec) + (int64_t)k * I.n + j);
+            re += x.re * z[k];
+            im += x.im * z[k];
         }
-        stv(C + j, cmake(re, im));
+        stv(gp(I.Cout) + j, cmake(re, im));
     }
-}
-
-// ---------------------------------------------------------------- host --
-
-struct LanczosBuffers {
-    LanczosState *st = nullptr;
-    cplx *HC = nullptr, *vin = nullptr, *diag = nullptr, *offd = nullptr, *ortho = nullptr;
-    double *partial = nullptr, *partial_o = nullptr;
-    int nb = 0, nbg = 0;
-    ~LanczosBuffers()
-    {
-        void *p[] = {st, HC, vin, diag, offd, ortho, partial, partial_o};
-        for (void *x : p)
-            if (x) (void)hipFree(x);
-    }
-};
-
-static int vec_blocks(int64_t n)
-{
-    int64_t b = (n + LZ_T * 4 - 1) / (LZ_T * 4);
-    return (int)std::max<int64_t>(1, std::min<int64_t>(b, 1024));
-}
-
-static mcpi_status lanczos_run(mcpi_ham h, int lm, const cplx *C0, cplx *lvec, cplx *diag_h, cplx *offd_h,
-                               int *steps_out, hipStream_t s)
-{
-    const int64_t n = h->basis->d.nc;
-    LanczosBuffers B;
-    B.nb = vec_blocks(n);
-    B.nbg = (int)((n + LZ_SEG - 1) / LZ_SEG);
-    MCPI_HIP(hipMalloc(&B.st, sizeof(LanczosState)));
-    MCPI_HIP(hipMalloc(&B.HC, sizeof(cplx) * n));
-    MCPI_HIP(hipMalloc(&B.vin, sizeof(cplx) * n));
-    MCPI_HIP(hipMalloc(&B.diag, sizeof(cplx) * lm));
-    MCPI_HIP(hipMalloc(&B.offd, sizeof(cplx) * lm));
-    MCPI_HIP(hipMalloc(&B.ortho, sizeof(cplx) * lm));
-    MCPI_HIP(hipMalloc(&B.partial, sizeof(double) * 2 * B.nb));
-    MCPI_HIP(hipMalloc(&B.partial_o, sizeof(double) * 2 * (size_t)B.nbg * lm));
-    std::vector<cplx> zeros(lm, cmake(0, 0));
-    MCPI_HIP(hipMemcpyAsync(B.diag, zeros.data(), sizeof(cplx) * lm, hipMemcpyHostToDevice, s));
-    MCPI_HIP(hipMemcpyAsync(B.offd, zeros.data(), sizeof(cplx) * lm, hipMemcpyHostToDevice, s));
-
-    // set-up: v_0, HC = H v_0, alpha_0, HC -= alpha_0 v_0, ||HC||^2 partials
-    hipLaunchKernelGGL(lz_init, dim3(1), dim3(1), 0, s, B.st, lm, 1e-15);
-    if (lvec != C0) MCPI_HIP(hipMemcpyAsync(lvec, C0, sizeof(cplx) * n, hipMemcpyDeviceToDevice, s));
-    MCPI_HIP(hipMemcpyAsync(B.vin, C0, sizeof(cplx) * n, hipMemcpyDeviceToDevice, s));
-    MCPI_CHECK(launch_apply(h, B.vin, B.HC, s));
-    hipLaunchKernelGGL(lz_dot, dim3(B.nb), dim3(LZ_T), 0, s, n, (const cplx *)B.vin, (const cplx *)B.HC, B.partial);
-    hipLaunchKernelGGL(lz_fin_alpha, dim3(1), dim3(LZ_T), 0, s, B.st, B.nb, (const double *)B.partial, B.diag, 1);
-    hipLaunchKernelGGL(lz_sub_alpha_norm, dim3(B.nb), dim3(LZ_T), 0, s, n, (const LanczosState *)B.st,
-                       (const cplx *)B.vin, B.HC, B.partial);
-    MCPI_HIP(hipGetLastError());
-
-    if (lm > 1) {
-        auto enqueue = [&]() -> mcpi_status {
-            hipLaunchKernelGGL(lz_fin_beta, dim3(1), dim3(LZ_T), 0, s, B.st, B.nb, (const double *)B.partial, B.offd);
-            hipLaunchKernelGGL(lz_scale, dim3(B.nb), dim3(LZ_T), 0, s, n, (const LanczosState *)B.st,
-                               (const cplx *)B.HC, lvec, B.vin);
-            MCPI_CHECK(launch_apply(h, B.vin, B.HC, s));
-            hipLaunchKernelGGL(lz_sub_dot, dim3(B.nb), dim3(LZ_T), 0, s, n, (const LanczosState *)B.st,
-                               (const cplx *)lvec, (const cplx *)B.vin, B.HC, B.partial);
-            hipLaunchKernelGGL(lz_fin_alpha, dim3(1), dim3(LZ_T), 0, s, B.st, B.nb, (const double *)B.partial, B.diag, 0);
-            hipLaunchKernelGGL(lz_sub_alpha, dim3(B.nb), dim3(LZ_T), 0, s, n, (const LanczosState *)B.st,
-                               (const cplx *)B.vin, B.HC);
-            hipLaunchKernelGGL(lz_gemv_dots, dim3(B.nbg, (lm + LZ_TG - 1) / LZ_TG), dim3(LZ_T), 0, s, n,
-                               (const LanczosState *)B.st, (const cplx *)lvec, (const cplx *)B.HC,
-                               B.partial_o, lm);
-            hipLaunchKernelGGL(lz_fin_ortho, dim3(1), dim3(LZ_T), 0, s, (const LanczosState *)B.st, B.nbg,
-                               (const double *)B.partial_o, lm, B.ortho);
-            hipLaunchKernelGGL(lz_reorth_norm, dim3(B.nb), dim3(LZ_T), sizeof(double) * 2 * lm, s, n,
-                               (const LanczosState *)B.st, (const cplx *)lvec, (const cplx *)B.ortho, B.HC,
-                               B.partial);
-            hipLaunchKernelGGL(lz_advance, dim3(1), dim3(1), 0, s, B.st);
-            MCPI_HIP(hipGetLastError());
-            return MCPI_OK;
-        };
-        // capture one iteration and replay it; fall back to plain launches
-        // if the runtime cannot capture on this stream
-        hipGraph_t graph = nullptr;
-        hipGraphExec_t exec = nullptr;
-        bool graphed = false;
-        if (s != nullptr && hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal) == hipSuccess) {
-            mcpi_status st = enqueue();
-            hipError_t e = hipStreamEndCapture(s, &graph);
-            if (st == MCPI_OK && e == hipSuccess && hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0) == hipSuccess)
-                graphed = true;
-            else
-                (void)hipGetLastError();
-        }
-        if (graphed) {
-            for (int i = 0; i < lm - 1; i++) MCPI_HIP(hipGraphLaunch(exec, s));
-            MCPI_HIP(hipGraphExecDestroy(exec));
-            MCPI_HIP(hipGraphDestroy(graph));
-        } else {
-            if (graph) (void)hipGraphDestroy(graph);
-            for (int i = 0; i < lm - 1; i++) MCPI_CHECK(enqueue());
-        }
-    }
-    LanczosState hs;
-    MCPI_HIP(hipMemcpyAsync(&hs, B.st, sizeof hs, hipMemcpyDeviceToHost, s));
-    MCPI_HIP(hipMemcpyAsync(diag_h, B.diag, sizeof(cplx) * lm, hipMemcpyDeviceToHost, s));
-    MCPI_HIP(hipMemcpyAsync(offd_h, B.offd, sizeof(cplx) * lm, hipMemcpyDeviceToHost, s));
-    MCPI_HIP(hipStreamSynchronize(s));
-    *steps_out = hs.broke ? hs.steps : lm;
-    if (!hs.broke && lm >= 1) offd_h[lm - 1] = cmake(0, 0);
-    return MCPI_OK;
 }
 
 // ------------------------------------------- tridiagonal eigenproblem --
@@ -578,51 +560,370 @@ mcpi_status tridiag_lowest(int n, const double *d, const double *e, double *lamb
     return MCPI_OK;
 }
 
+// ---------------------------------------------------------------- host --
+
+static int vec_blocks(int64_t n)
+{
+    int64_t b = (n + LZ_T * 4 - 1) / (LZ_T * 4);
+    return (int)std::max<int64_t>(1, std::min<int64_t>(b, 1024));
+}
+
+// One device allocation carved into aligned pieces.
+struct Arena {
+    char *base = nullptr;
+    size_t used = 0, cap = 0;
+    bool overflow = false;
+    ~Arena()
+    {
+        if (base) (void)hipFree(base);
+    }
+    static size_t up(size_t x) { return (x + 255) & ~(size_t)255; }
+    void reserve(size_t bytes) { cap += up(bytes); }
+    // pieces must be taken in the sizes they were reserved in
+    template <class T>
+    T *take(size_t count)
+    {
+        if (used + up(sizeof(T) * count) > cap) {
+            overflow = true;
+            return nullptr;
+        }
+        T *p = (T *)(base + used);
+        used += up(sizeof(T) * count);
+        return p;
+    }
+    mcpi_status allocate()
+    {
+        hipError_t e = hipMalloc((void **)&base, cap ? cap : 256);
+        if (e != hipSuccess) {
+            (void)hipGetLastError();
+            set_error("cannot allocate the Lanczos workspace (lm * nc complex per instance)");
+            return MCPI_ERR_ALLOC;
+        }
+        return MCPI_OK;
+    }
+};
+
+// Instances that share a basis: H.v for all of them is one launch of the
+// specialised kernel when they agree on zero-skipping and schedule and a
+// specialisation exists; otherwise each instance is launched on its own.
+// A batched group keeps its members' coefficient blocks and H.v input and
+// output vectors side by side at fixed strides (the batched kernel's layout).
+struct HvGroup {
+    mcpi_basis basis = nullptr;
+    std::vector<int> members;
+    bool batched = false;
+    int sched = 2, skip = 0;
+    size_t cbytes = 0;            // coefficient block size = stride
+    HamCoefDev c0{};              // member 0's tables inside `coefs`
+    char *coefs = nullptr;        // members' coefficient blocks, cbytes apart
+    cplx *vin = nullptr, *HC = nullptr;   // members' vectors, nc apart
+};
+
+static mcpi_status launch_group(const HvGroup &g, const mcpi_ham *hams, const std::vector<LzInst> &hin,
+                                hipStream_t s)
+{
+    if (g.batched)
+        return launch_apply_batch(g.basis->d, g.basis->maps, g.c0, g.skip, g.sched, (int)g.members.size(), g.cbytes,
+                                  g.vin, g.HC, g.basis->d.nc, s);
+    for (int x : g.members) MCPI_CHECK(launch_apply(hams[x], hin[x].vin, hin[x].HC, s));
+    return MCPI_OK;
+}
+
+// time the specialised schedules on the whole group, keep the faster
+static mcpi_status tune_group(HvGroup &g, const mcpi_ham *hams, const std::vector<LzInst> &hin, hipStream_t s)
+{
+    std::vector<int> cand{2};
+    if (g.basis->maps == MCPI_MAPS_FULL) cand.push_back(3);
+    if (cand.size() == 1) {
+        g.sched = cand[0];
+        return MCPI_OK;
+    }
+    hipEvent_t e0, e1;
+    MCPI_HIP(hipEventCreate(&e0));
+    MCPI_HIP(hipEventCreate(&e1));
+    float best = 0;
+    int pick = cand[0];
+    mcpi_status rc = MCPI_OK;
+    for (int c : cand) {
+        g.sched = c;
+        rc = launch_group(g, hams, hin, s);   // warm-up
+        if (rc != MCPI_OK) break;
+        if (hipEventRecord(e0, s) != hipSuccess) { rc = MCPI_ERR_HIP; break; }
+        for (int r = 0; r < 2 && rc == MCPI_OK; r++) rc = launch_group(g, hams, hin, s);
+        if (rc != MCPI_OK || hipEventRecord(e1, s) != hipSuccess || hipEventSynchronize(e1) != hipSuccess) {
+            if (rc == MCPI_OK) rc = MCPI_ERR_HIP;
+            break;
+        }
+        float ms = 0;
+        (void)hipEventElapsedTime(&ms, e0, e1);
+        if (c == cand[0] || ms < best) { best = ms; pick = c; }
+    }
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    if (rc == MCPI_ERR_HIP) set_error("H.v schedule timing failed");
+    g.sched = pick;
+    return rc;
+}
+
+// K instances of length lm.  lvec_ext[x] (may be null, or the array null):
+// caller's lm*n basis buffer.  diag_h/offd_h: host, K*lm (row x = instance
+// x).  Cout (may be null, or entries null): Ritz vectors; E0 (may be null
+// only when Cout is): lowest eigenvalues.  steps: K.
+static mcpi_status lanczos_core(int K, const mcpi_ham *hams, const cplx *const *C0, cplx *const *lvec_ext, int lm,
+                                cplx *diag_h, cplx *offd_h, int *steps, cplx *const *Cout, double *E0, hipStream_t s)
+{
+    MCPI_ARG(K >= 0 && K <= 65535, "number of instances out of range");
+    MCPI_ARG(lm >= 1 && lm <= (1 << 20), "Lanczos length must be in [1, 2^20]");
+    if (K == 0) return MCPI_OK;
+    const bool want_e = E0 != nullptr;
+
+    // ---- H.v groups by basis
+    std::vector<HvGroup> groups;
+    std::vector<int> batched_member(K, -1);   // index within a batched group
+    for (int x = 0; x < K; x++) {
+        MCPI_ARG(hams[x] && C0[x], "null instance");
+        auto it = std::find_if(groups.begin(), groups.end(),
+                               [&](const HvGroup &g) { return g.basis == hams[x]->basis; });
+        if (it == groups.end()) {
+            groups.emplace_back();
+            groups.back().basis = hams[x]->basis;
+            it = groups.end() - 1;
+        }
+        it->members.push_back(x);
+    }
+    for (HvGroup &g : groups) {
+        const mcpi_ham h0 = hams[g.members[0]];
+        bool ok = g.members.size() > 1 && has_fast_kernel(g.basis->d.N, g.basis->d.M);
+        bool real = true;
+        for (int x : g.members) {
+            const mcpi_ham h = hams[x];
+            ok = ok && h->skip_zero == h0->skip_zero && h->schedule == h0->schedule && h->schedule != 1 &&
+                 h->coef_bytes == h0->coef_bytes;
+            real = real && h->c.real;
+        }
+        g.batched = ok;
+        if (ok) {
+            g.skip = h0->skip_zero;
+            g.cbytes = h0->coef_bytes;
+            g.c0.real = real ? 1 : 0;
+            for (size_t k = 0; k < g.members.size(); k++) batched_member[g.members[k]] = (int)k;
+        }
+    }
+
+    // ---- workspace
+    Arena A;
+    A.reserve(sizeof(LanczosState) * K);
+    A.reserve(sizeof(LzInst) * K);
+    std::vector<LzInst> hin(K);
+    int nbmax = 1, nbgmax = 1;
+    for (int x = 0; x < K; x++) {
+        const int64_t n = hams[x]->basis->d.nc;
+        LzInst &I = hin[x];
+        I.n = n;
+        I.nb = vec_blocks(n);
+        I.nbg = (int)((n + LZ_SEG - 1) / LZ_SEG);
+        nbmax = std::max(nbmax, I.nb);
+        nbgmax = std::max(nbgmax, I.nbg);
+        if (!(lvec_ext && lvec_ext[x])) A.reserve(sizeof(cplx) * n * (size_t)lm);
+        if (batched_member[x] < 0)
+            for (int k = 0; k < 2; k++) A.reserve(sizeof(cplx) * n);
+        for (int k = 0; k < 3; k++) A.reserve(sizeof(cplx) * lm);
+        A.reserve(sizeof(double) * 2 * I.nb);
+        A.reserve(sizeof(double) * 2 * (size_t)I.nbg * lm);
+        A.reserve(sizeof(double) * lm);
+    }
+    for (HvGroup &g : groups) {
+        if (!g.batched) continue;
+        const size_t m = g.members.size();
+        A.reserve(g.cbytes * m);
+        for (int k = 0; k < 2; k++) A.reserve(sizeof(cplx) * (size_t)g.basis->d.nc * m);
+    }
+    MCPI_CHECK(A.allocate());
+    LanczosState *sts = A.take<LanczosState>(K);
+    LzInst *din = A.take<LzInst>(K);
+    for (HvGroup &g : groups) {
+        if (!g.batched) continue;
+        const size_t m = g.members.size();
+        g.coefs = A.take<char>(g.cbytes * m);
+        g.vin = A.take<cplx>((size_t)g.basis->d.nc * m);
+        g.HC = A.take<cplx>((size_t)g.basis->d.nc * m);
+        if (A.overflow) break;
+        for (size_t k = 0; k < m; k++) {
+            const mcpi_ham h = hams[g.members[k]];
+            MCPI_HIP(hipMemcpyAsync(g.coefs + k * g.cbytes, h->coef_block, g.cbytes, hipMemcpyDeviceToDevice, s));
+        }
+        const mcpi_ham h0 = hams[g.members[0]];
+        const int real = g.c0.real;
+        g.c0 = shift_coef(h0->c, (ptrdiff_t)((uintptr_t)g.coefs - (uintptr_t)h0->coef_block));
+        g.c0.real = real;
+    }
+    for (int x = 0; x < K; x++) {
+        LzInst &I = hin[x];
+        const int64_t n = I.n;
+        I.C0 = C0[x];
+        I.lvec = (lvec_ext && lvec_ext[x]) ? lvec_ext[x] : A.take<cplx>(n * (size_t)lm);
+        if (batched_member[x] < 0) {
+            I.HC = A.take<cplx>(n);
+            I.vin = A.take<cplx>(n);
+        }
+        I.diag = A.take<cplx>(lm);
+        I.offd = A.take<cplx>(lm);
+        I.ortho = A.take<cplx>(lm);
+        I.partial = A.take<double>(2 * I.nb);
+        I.partial_o = A.take<double>(2 * (size_t)I.nbg * lm);
+        double *z = A.take<double>(lm);
+        I.z = z;
+        I.Cout = (Cout && Cout[x]) ? Cout[x] : nullptr;
+        I.steps = 0;
+    }
+    for (HvGroup &g : groups) {
+        if (!g.batched || !g.vin) continue;
+        for (size_t k = 0; k < g.members.size(); k++) {
+            hin[g.members[k]].vin = g.vin + k * (size_t)g.basis->d.nc;
+            hin[g.members[k]].HC = g.HC + k * (size_t)g.basis->d.nc;
+        }
+    }
+    if (A.overflow) {
+        set_error("internal error: Lanczos workspace layout");
+        return MCPI_ERR_STATE;
+    }
+    MCPI_HIP(hipMemcpyAsync(din, hin.data(), sizeof(LzInst) * K, hipMemcpyHostToDevice, s));
+
+    const dim3 blk(LZ_T), gv(nbmax, K), g1(1, K);
+    const dim3 gd(nbgmax, (lm + LZ_TG - 1) / LZ_TG, K);
+    const int ga = (K + 63) / 64;
+
+    // ---- set-up: v_0, schedules, HC = H v_0, alpha_0, HC -= alpha_0 v_0
+    hipLaunchKernelGGL(lz_init, g1, dim3(64), 0, s, sts, (const LzInst *)din, lm, 1e-15);
+    hipLaunchKernelGGL(lz_start, gv, blk, 0, s, (const LzInst *)din);
+    MCPI_HIP(hipGetLastError());
+    for (HvGroup &g : groups) {
+        if (!g.batched) continue;
+        const mcpi_ham h0 = hams[g.members[0]];
+        if (h0->schedule == 2 || h0->schedule == 3) g.sched = h0->schedule;
+        else MCPI_CHECK(tune_group(g, hams, hin, s));
+    }
+    for (const HvGroup &g : groups) MCPI_CHECK(launch_group(g, hams, hin, s));
+    hipLaunchKernelGGL(lz_dot, gv, blk, 0, s, (const LzInst *)din);
+    hipLaunchKernelGGL(lz_fin_alpha, g1, blk, 0, s, sts, (const LzInst *)din, 1);
+    hipLaunchKernelGGL(lz_sub_alpha_norm, gv, blk, 0, s, (const LanczosState *)sts, (const LzInst *)din);
+    MCPI_HIP(hipGetLastError());
+
+    if (lm > 1) {
+        const LanczosState *cst = sts;
+        const LzInst *cin = din;
+        auto enqueue = [&]() -> mcpi_status {
+            hipLaunchKernelGGL(lz_fin_beta, g1, blk, 0, s, sts, cin);
+            hipLaunchKernelGGL(lz_scale, gv, blk, 0, s, cst, cin);
+            for (const HvGroup &g : groups) MCPI_CHECK(launch_group(g, hams, hin, s));
+            hipLaunchKernelGGL(lz_sub_dot, gv, blk, 0, s, cst, cin);
+            hipLaunchKernelGGL(lz_fin_alpha, g1, blk, 0, s, sts, cin, 0);
+            hipLaunchKernelGGL(lz_sub_alpha, gv, blk, 0, s, cst, cin);
+            hipLaunchKernelGGL(lz_gemv_dots, gd, blk, 0, s, cst, cin, lm);
+            hipLaunchKernelGGL(lz_fin_ortho, g1, blk, 0, s, cst, cin, lm);
+            hipLaunchKernelGGL(lz_reorth_norm, gv, blk, 0, s, cst, cin);
+            hipLaunchKernelGGL(lz_advance, dim3(ga), dim3(64), 0, s, sts, K);
+            MCPI_HIP(hipGetLastError());
+            return MCPI_OK;
+        };
+        // capture one iteration and replay it; fall back to plain launches
+        // if the runtime cannot capture on this stream
+        hipGraph_t graph = nullptr;
+        hipGraphExec_t exec = nullptr;
+        bool graphed = false;
+        if (s != nullptr && hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal) == hipSuccess) {
+            mcpi_status st = enqueue();
+            hipError_t e = hipStreamEndCapture(s, &graph);
+            if (st == MCPI_OK && e == hipSuccess && hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0) == hipSuccess)
+                graphed = true;
+            else
+                (void)hipGetLastError();
+        }
+        if (graphed) {
+            mcpi_status rc = MCPI_OK;
+            for (int i = 0; i < lm - 1 && rc == MCPI_OK; i++) {
+                hipError_t e = hipGraphLaunch(exec, s);
+                if (e != hipSuccess) rc = hip_fail(e, "hipGraphLaunch", __FILE__, __LINE__);
+            }
+            (void)hipGraphExecDestroy(exec);
+            (void)hipGraphDestroy(graph);
+            MCPI_CHECK(rc);
+        } else {
+            if (graph) (void)hipGraphDestroy(graph);
+            for (int i = 0; i < lm - 1; i++) MCPI_CHECK(enqueue());
+        }
+    }
+
+    // ---- results to the host
+    std::vector<LanczosState> hs(K);
+    std::vector<cplx> dg((size_t)K * lm), od((size_t)K * lm);
+    MCPI_HIP(hipMemcpyAsync(hs.data(), sts, sizeof(LanczosState) * K, hipMemcpyDeviceToHost, s));
+    for (int x = 0; x < K; x++) {
+        MCPI_HIP(hipMemcpyAsync(&dg[(size_t)x * lm], hin[x].diag, sizeof(cplx) * lm, hipMemcpyDeviceToHost, s));
+        MCPI_HIP(hipMemcpyAsync(&od[(size_t)x * lm], hin[x].offd, sizeof(cplx) * lm, hipMemcpyDeviceToHost, s));
+    }
+    MCPI_HIP(hipStreamSynchronize(s));
+    for (int x = 0; x < K; x++) {
+        const int st = hs[x].broke ? hs[x].steps : lm;
+        steps[x] = st;
+        if (!hs[x].broke) od[(size_t)x * lm + lm - 1] = cmake(0, 0);
+    }
+    if (diag_h) std::memcpy(diag_h, dg.data(), sizeof(cplx) * K * lm);
+    if (offd_h) std::memcpy(offd_h, od.data(), sizeof(cplx) * K * lm);
+    if (!want_e) return MCPI_OK;
+
+    // ---- tridiagonal eigenproblems on the host, Ritz vectors on the device
+    bool any_ritz = false;
+    std::vector<double> zall((size_t)K * lm, 0.0);
+    for (int x = 0; x < K; x++) {
+        const int n = steps[x];
+        std::vector<double> d(n), e(n);
+        for (int k = 0; k < n; k++) {
+            d[k] = dg[(size_t)x * lm + k].re;
+            e[k] = od[(size_t)x * lm + k].re;
+        }
+        MCPI_CHECK(tridiag_lowest(n, d.data(), e.data(), &E0[x], hin[x].Cout ? &zall[(size_t)x * lm] : nullptr));
+        hin[x].steps = n;
+        any_ritz = any_ritz || hin[x].Cout;
+    }
+    if (any_ritz) {
+        for (int x = 0; x < K; x++)
+            if (hin[x].Cout)
+                MCPI_HIP(hipMemcpyAsync((void *)hin[x].z, &zall[(size_t)x * lm], sizeof(double) * lm,
+                                        hipMemcpyHostToDevice, s));
+        MCPI_HIP(hipMemcpyAsync(din, hin.data(), sizeof(LzInst) * K, hipMemcpyHostToDevice, s));
+        hipLaunchKernelGGL(lz_ritz, gv, blk, 0, s, (const LzInst *)din);
+        MCPI_HIP(hipGetLastError());
+        MCPI_HIP(hipStreamSynchronize(s));
+    }
+    return MCPI_OK;
+}
+
 mcpi_status ground_run(mcpi_ham h, int lm, const cplx *C0, cplx *Cout, double *E0, int *steps_out,
                        hipStream_t s, cplx *lvec_ext)
 {
-    const int64_t n = h->basis->d.nc;
-    MCPI_ARG(lm >= 1, "Lanczos length must be >= 1");
-    cplx *lvec = lvec_ext;
-    if (!lvec) {
-        hipError_t e = hipMalloc(&lvec, sizeof(cplx) * n * (size_t)lm);
-        if (e != hipSuccess) {
-            (void)hipGetLastError();
-            set_error("cannot allocate the Lanczos basis (lm * nc complex)");
-            return MCPI_ERR_ALLOC;
-        }
-    }
-    std::vector<cplx> diag(lm), offd(lm);
-    int steps = 0;
-    mcpi_status st = lanczos_run(h, lm, C0, lvec, diag.data(), offd.data(), &steps, s);
-    if (st == MCPI_OK) {
-        std::vector<double> d(steps), e(steps), z(steps);
-        for (int k = 0; k < steps; k++) { d[k] = diag[k].re; e[k] = offd[k].re; }
-        double lam = 0;
-        st = tridiag_lowest(steps, d.data(), e.data(), &lam, z.data());
-        if (st == MCPI_OK) {
-            double *zd = nullptr;
-            hipError_t err = hipMalloc(&zd, sizeof(double) * steps);
-            if (err == hipSuccess) err = hipMemcpyAsync(zd, z.data(), sizeof(double) * steps, hipMemcpyHostToDevice, s);
-            if (err == hipSuccess) {
-                hipLaunchKernelGGL(lz_ritz, dim3(vec_blocks(n)), dim3(LZ_T), sizeof(double) * steps, s, n, steps,
-                                   (const cplx *)lvec, (const double *)zd, Cout);
-                err = hipStreamSynchronize(s);
-            }
-            if (zd) (void)hipFree(zd);
-            if (err != hipSuccess) st = hip_fail(err, "Ritz vector", __FILE__, __LINE__);
-            *E0 = lam;
-            if (steps_out) *steps_out = steps;
-        }
-    }
-    if (!lvec_ext) (void)hipFree(lvec);
-    return st;
+    MCPI_ARG(h && C0 && Cout && E0, "bad argument");
+    int st = 0;
+    cplx *const lv[1] = {lvec_ext};
+    cplx *const co[1] = {Cout};
+    MCPI_CHECK(lanczos_core(1, &h, &C0, lv, lm, nullptr, nullptr, &st, co, E0, s));
+    if (steps_out) *steps_out = st;
+    return MCPI_OK;
 }
 
 mcpi_status lanczos_api(mcpi_ham h, int lm, const cplx *C0, cplx *lvec, cplx *diag, cplx *offd, int *steps,
                         hipStream_t s)
 {
-    return lanczos_run(h, lm, C0, lvec, diag, offd, steps, s);
+    MCPI_ARG(h && C0 && lvec && diag && offd && steps, "bad argument");
+    cplx *const lv[1] = {lvec};
+    return lanczos_core(1, &h, &C0, lv, lm, diag, offd, steps, nullptr, nullptr, s);
+}
+
+mcpi_status lanczos_batched_api(int K, const mcpi_ham *hams, const cplx *const *C0, int lm, double *E0,
+                                int *steps, cplx *const *Cout, hipStream_t s)
+{
+    MCPI_ARG(K >= 0 && (K == 0 || (hams && C0 && E0 && steps)), "bad argument");
+    return lanczos_core(K, hams, C0, nullptr, lm, nullptr, nullptr, steps, Cout, E0, s);
 }
 
 }  // namespace mcpi

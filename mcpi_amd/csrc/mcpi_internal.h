@@ -132,10 +132,34 @@ struct HamCoefDev {
     int real;        // 1 if every coefficient above has a zero imaginary part
 };
 
+// the same tables d bytes further on (instance x of a batch, or a copy of
+// the block elsewhere)
+__host__ __device__ inline HamCoefDev shift_coef(HamCoefDev c, ptrdiff_t d)
+{
+    auto sh = [d](auto *p) { return (decltype(p))((char *)p + d); };
+    c.dk = sh(c.dk);
+    c.d1 = sh(c.d1);
+    c.d2 = sh(c.d2);
+    c.A0 = sh(c.A0);
+    c.A = sh(c.A);
+    c.ot = sh(c.ot);
+    c.tt = sh(c.tt);
+    c.nz_1j = sh(c.nz_1j);
+    c.nz_ot = sh(c.nz_ot);
+    c.nz_tt = sh(c.nz_tt);
+    c.m1j = sh(c.m1j);
+    c.ot_off = sh(c.ot_off);
+    c.tt_off = sh(c.tt_off);
+    c.nz_e = sh(c.nz_e);
+    c.nz_c = sh(c.nz_c);
+    return c;
+}
+
 struct mcpi_ham_s {
     mcpi_basis basis;
     HamCoefDev c;
     void *coef_block;   // single device allocation holding the tables
+    size_t coef_bytes;  // its size (the same for every Hamiltonian with M orbitals)
     cplx *Ho_dev;       // M*M flat, k + M*l
     cplx *Hint_dev;     // M^4
     int skip_zero;
@@ -154,12 +178,15 @@ __host__ __device__ inline int pair_index(int k, int s, int M)
 // kernels launched from host code of other translation units
 mcpi_status launch_apply(mcpi_ham h, const cplx *C, cplx *out, hipStream_t st);
 int ham_schedule(mcpi_ham h);
+bool has_fast_kernel(int N, int M);
+mcpi_status launch_apply_batch(const BasisDev &b, int maps, const HamCoefDev &c0, int skip, int sched, int nbatch,
+                               size_t cstride, const cplx *C0, cplx *out0, int64_t vstride, hipStream_t st);
 // schedules: 1 generic thread-per-row kernel, 2 specialised kernel reading
 // every stored map, 3 specialised kernel ranking double jumps on the fly
 mcpi_status launch_apply_raw(const BasisDev &b, int maps, const HamCoefDev &c, int skip_zero,
                              int schedule, const cplx *C, cplx *out, hipStream_t st);
-mcpi_status build_coefficients(int M, const cplx *Ho, const cplx *Hint, HamCoefDev *c,
-                               void **block);
+mcpi_status build_coefficients(int M, const cplx *Ho, const cplx *Hint, HamCoefDev *c, void **block,
+                               size_t *block_bytes = nullptr);
 mcpi_status basis_upload_raw(int N, int M, const int32_t *IF, const int32_t *NCmat,
                              const int32_t *Map, const int32_t *MapOT, const int32_t *MapTT,
                              const int *strideOT, const int *strideTT, int64_t ot_len,
