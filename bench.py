@@ -152,6 +152,49 @@ def batched_lanczos(n_inst: int = 64, lm: int = 120, seed: int = 64120, serial: 
     }
 
 
+def large_basis(reps: int = 20, lm: int = 200):
+    """N=16 bosons, M=12 orbitals (nc = 13,037,895), Lieb-Liniger g=50, only
+    the single-jump Map stored (7.5 GB; the double-jump maps would be 200 GB)
+    and double jumps ranked on the fly.  H.v timed with every term evaluated
+    and with zero-skipping; then a 200-step Lanczos ground state from the
+    reference's start vector (groundStateLieb.c:578-586) and the adjoint
+    (rho1, rho2 / 2) at the ground vector."""
+    import mcpi_amd
+    N, M, g = 16, 12, 50.0
+    t0 = time.time()
+    b = mcpi_amd.Basis(N, M, "single")
+    build_s = time.time() - t0
+    nc = b.nc
+    B = 16 * nc + 16 * nc + 4 * M * nc + 4 * M * M * nc
+    rng = np.random.default_rng(1612)
+    C = rng.standard_normal(nc) + 1j * rng.standard_normal(nc)
+    dC = mcpi_amd.DeviceArray.from_numpy(C / np.linalg.norm(C))
+    dO = mcpi_amd.DeviceArray(nc)
+    Ho, Hint = mcpi_amd.ll_ho(M), mcpi_amd.ll_hint(M, g)
+    out = {"workload": "N=16, M=12, Lieb-Liniger g=50, Map stored, double jumps ranked",
+           "nc": nc, "bytes_per_hv": B, "basis_build_s": round(build_s, 3)}
+    for label, skip in (("every_term", False), ("skip_zero", True)):
+        H = mcpi_amd.Hamiltonian(b, Ho, Hint, skip_zero=skip)
+        H.apply(dC, dO)                      # schedule tuning
+        ms = H.apply_timed(dC, dO, reps)
+        out[label] = {"ms_per_hv": round(ms, 4), "gb_s_model": round(B / ms / 1e6, 1),
+                      "hbm_frac_model": round(B / ms / 1e6 / HBM_PEAK_GBS, 4), "schedule": H.schedule}
+        H.close()
+    del dO
+    H = mcpi_amd.Hamiltonian(b, Ho, Hint)
+    dS = mcpi_amd.DeviceArray.from_numpy(mcpi_amd.start_vector(nc))
+    t = time.perf_counter()
+    E0, psi, steps = H.ground(lm, dS)
+    t_ground = time.perf_counter() - t
+    t = time.perf_counter()
+    g1, g2 = b.vjp(psi)
+    t_vjp = time.perf_counter() - t
+    out["ground"] = {"lm": lm, "steps": steps, "E0_per_N": E0 / N, "s": round(t_ground, 3),
+                     "vjp_s": round(t_vjp, 3),
+                     "energy_identity_residual": float(abs(np.real(np.sum(g1 * Ho) + np.sum(g2 * Hint)) - E0))}
+    return out
+
+
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -166,6 +209,8 @@ def main() -> int:
     ap.add_argument("--skip-zero", action="store_true",
                     help="time the zero-skipping path instead (not the headline)")
     ap.add_argument("--no-batched", action="store_true", help="skip the batched Lanczos comparison")
+    ap.add_argument("--no-large", action="store_true", help="skip the N=16, M=12 measurements")
+    ap.add_argument("--large-only", action="store_true", help="only run the N=16, M=12 measurements")
     ap.add_argument("--batched-only", action="store_true",
                     help="only run the batched Lanczos call, without the per-instance solves "
                          "(profiling runs)")
@@ -190,6 +235,9 @@ def main() -> int:
         print(json.dumps({"copy_calibration": {"bytes_per_copy": n, "copies": 3}}))
         return 0
 
+    if args.large_only:
+        print(json.dumps({"large_basis": large_basis()}))
+        return 0
     if args.batched_only:   # the batched call alone (no per-instance solves)
         print(json.dumps({"batched_lanczos": batched_lanczos(serial=False)}))
         return 0
@@ -237,7 +285,7 @@ def main() -> int:
             others[label] = {"ms_per_hv": round(Hx.apply_timed(dC, dO, reps), 4), "schedule": Hx.schedule}
             agree = max(agree, float(np.linalg.norm(dO.numpy() - gpu_out) / np.linalg.norm(gpu_out)))
 
-    batched = None
+    batched = large = None
     if not (args.no_batched or args.profile):
         del dO
         batched = batched_lanczos()
@@ -245,6 +293,8 @@ def main() -> int:
     cpu = None
     if not (args.no_cpu or args.profile):
         cpu = cpu_baseline(b, C, Ho, Hint, gpu_out, args.cpu_seconds)
+    if not (args.no_large or args.profile):
+        large = large_basis()
 
     line = {
         "metric": "H·v GB/s (% HBM peak)",
@@ -282,6 +332,7 @@ def main() -> int:
             "basis_build_s": round(build_s, 3),
             "other_schedules": others,
             "batched_lanczos": batched,
+            "large_basis_n16_m12": large,
             "max_rel_diff_between_schedules": agree,
             "speedup_vs_cpu": (round(cpu["ms_per_hv"] / ms_step, 1) if cpu else None),
         },

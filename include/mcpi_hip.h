@@ -150,6 +150,9 @@ typedef struct {
     int64_t map_len;         /* nc*M*M or 0                                 */
     int64_t ot_len, tt_len;  /* |MapOT|, |MapTT| in ints (0 if not stored)  */
     int64_t device_bytes;    /* device memory held by the basis             */
+    int64_t pot_len, ptt_len; /* ints of the wave-packed OT / TT tables that
+                                 the H.v kernels read (full maps only; the
+                                 layout is given at mcpi_hip_basis_copy_range) */
 } mcpi_basis_info;
 
 const char *mcpi_hip_last_error(void);
@@ -177,7 +180,16 @@ mcpi_status mcpi_hip_basis_copy_map(mcpi_basis b, int32_t *Map);
 mcpi_status mcpi_hip_basis_copy_ot(mcpi_basis b, int64_t *strideOT, int32_t *MapOT);
 mcpi_status mcpi_hip_basis_copy_tt(mcpi_basis b, int64_t *strideTT, int32_t *MapTT);
 /* count entries of one table from offset (which: 0 Map, 1 MapOT, 2 MapTT,
- * 3 IF, 4 strideOT, 5 strideTT; strides are int64, the others int32) */
+ * 3 IF, 4 strideOT, 5 strideTT, 6 packOT, 7 packTT, 8 poffOT, 9 poffTT;
+ * strides and packed offsets are int64, the others int32).
+ * Wave-packed tables (full maps): wavefront w = rows 64w .. 64w+63; for each
+ * jump active in at least one of its rows (OT: k with v_k >= 2, ascending;
+ * TT: pairs k < s both occupied, lexicographic) one slot of
+ * E = 4*ceil(M(M+1)/8) entries per active row, starting at int offset
+ * poff[w] and following each other: slot[g][a][u] = target of entry
+ * t = 4g + u of the a-th active row (t = l(l+1)/2 + q, q <= l; entries
+ * t >= M(M+1)/2 hold the row's own index).  The target of (l, q) is the
+ * reference map's entry l + q*M of that chunk. */
 mcpi_status mcpi_hip_basis_copy_range(mcpi_basis b, int which, int64_t offset,
                                       int64_t count, void *dst);
 /* rank / unrank of many states on the GPU: v is n*M ints (host) */

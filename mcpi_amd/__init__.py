@@ -48,7 +48,8 @@ class MCPIError(RuntimeError):
 class _BasisInfo(ct.Structure):
     _fields_ = [("N", ct.c_int), ("M", ct.c_int), ("maps", ct.c_int),
                 ("nc", ct.c_int64), ("map_len", ct.c_int64), ("ot_len", ct.c_int64),
-                ("tt_len", ct.c_int64), ("device_bytes", ct.c_int64)]
+                ("tt_len", ct.c_int64), ("device_bytes", ct.c_int64),
+                ("pot_len", ct.c_int64), ("ptt_len", ct.c_int64)]
 
 
 _lib = None
@@ -204,6 +205,7 @@ class Basis:
         self.nc = int(info.nc)
         self.map_len, self.ot_len, self.tt_len = int(info.map_len), int(info.ot_len), int(info.tt_len)
         self.device_bytes = int(info.device_bytes)
+        self.pot_len, self.ptt_len = int(info.pot_len), int(info.ptt_len)
 
     def ncmat(self) -> np.ndarray:
         out = np.zeros((self.N + 1) * (self.M + 1), np.int32)
@@ -233,7 +235,9 @@ class Basis:
         return s, m[: self.tt_len]
 
     _WHICH = {"Map": (0, np.int32), "MapOT": (1, np.int32), "MapTT": (2, np.int32),
-              "IF": (3, np.int32), "strideOT": (4, np.int64), "strideTT": (5, np.int64)}
+              "IF": (3, np.int32), "strideOT": (4, np.int64), "strideTT": (5, np.int64),
+              "packOT": (6, np.int32), "packTT": (7, np.int32), "poffOT": (8, np.int64),
+              "poffTT": (9, np.int64)}
 
     def copy_range(self, table: str, offset: int, count: int) -> np.ndarray:
         """``count`` entries of a device table starting at ``offset``."""

@@ -159,6 +159,9 @@ mcpi_status build_coefficients(int M, const cplx *Ho, const cplx *Hint, HamCoefD
     for (const std::vector<cplx> *v : {&dk, &d1, &d2, &A0, &A, &ot, &tt})
         for (const cplx &z : *v) real = real && z.im == 0.0;
     c->real = real ? 1 : 0;
+    const int64_t listed = tt_off[P];   // every listed (non-zero) double-jump term
+    const int64_t half = (int64_t)(M + P) * (M * (M + 1) / 2);   // q <= l entries of every chunk kind
+    c->dense = (2 * listed >= half) ? 1 : 0;
     *block = dev;
     if (block_bytes) *block_bytes = bytes;
     return MCPI_OK;
@@ -361,7 +364,7 @@ mcpi_status launch_apply_raw(const BasisDev &b, int maps, const HamCoefDev &c, i
                              int schedule, const cplx *C, cplx *out, hipStream_t st)
 {
     if (schedule == 2 || schedule == 3) {
-        mcpi_status s = launch_fast(b, maps, c, skip_zero, schedule == 3, C, out, st);
+        mcpi_status s = launch_fast(b, maps, c, skip_zero && !c.dense, schedule == 3, C, out, st);
         if (s != MCPI_ERR_STATE) return s;
         set_error("no specialised H.v kernel for this number of orbitals");
         return MCPI_ERR_ARG;

@@ -19,10 +19,13 @@
 //         the kernel walks per-chunk lists of the non-zero terms instead of
 //         the whole chunk (result-neutral for finite inputs)
 //
-// Reading a double-jump chunk (DMAP): row l of a chunk holds entries
-// l*M .. l*M + l contiguously (the q <= l half actH uses); they are loaded as
-// whole 16-byte pieces, so every cache line is requested by consecutive
-// loads of one lane and fetched once.
+// Reading double-jump targets (DMAP): the wave-packed tables of the basis
+// (basis.hip) hold, per wavefront and active jump, the q <= l targets of the
+// active lanes interleaved in 16-byte groups, so one load instruction of the
+// wavefront reads one contiguous run instead of 64 separate cache lines.
+// The slot of a jump is found by a ballot over the lanes' activity: the
+// jumps are visited in the table's order and every active one advances the
+// wavefront's offset by (active lanes) x (entries per slot).
 //
 // Ranking on the fly: with prefix counts P_j = sum_{o<j} v_o,
 // rank(v) = nc - 1 - sum_{j=1}^{M-1} B[j][P_j] (closed form of the
@@ -171,36 +174,54 @@ __device__ inline uint32_t dl(const uint32_t *Dl, int lane, int T, int a, int x)
 // every entry q <= l of one chunk (coefficients of excluded entries are
 // zero; their targets are valid states, so no branch is needed).  Row l:
 // sum_{q<l} coef r1[q] C[j] is scaled once by r1[l]; q == l uses r2[l].
-template <int M, bool DMAP, bool RC>
-__device__ inline cplx chunk_all(const int32_t *__restrict__ chunk, const uint32_t (&D0)[M],
-                                 const uint32_t (&D1)[M], uint32_t base, const cplx *__restrict__ C,
-                                 const RowState<M> &r, const double *sq, const cplx *coef)
+// Targets ranked from the difference tables.
+template <int M, bool RC>
+__device__ inline cplx chunk_ranked(const uint32_t (&D0)[M], const uint32_t (&D1)[M], uint32_t base,
+                                    const cplx *__restrict__ C, const RowState<M> &r, const double *sq,
+                                    const cplx *coef)
 {
     cplx part = cmake(0, 0);
 #pragma unroll
     for (int l = 0; l < M; l++) {
-        const int e0 = l * M;
-        const int p0 = e0 / 4, p1 = (e0 + l) / 4;
-        int32_t val[8 * 4];
-        if (DMAP) {
-#pragma unroll
-            for (int pc = p0; pc <= p1; pc++) {
-                Piece4 t = *reinterpret_cast<const Piece4 *>(chunk + 4 * pc);
-#pragma unroll
-                for (int u = 0; u < 4; u++) val[4 * (pc - p0) + u] = t.x[u];
-            }
-        }
         cplx pl = cmake(0, 0);
 #pragma unroll
         for (int q = 0; q <= l; q++) {
-            const int e = e0 + q;
-            const int64_t j = DMAP ? (int64_t)val[e - 4 * p0] : (int64_t)(uint32_t)(base - D0[q] - D1[l]);
+            const int64_t j = (int64_t)(uint32_t)(base - D0[q] - D1[l]);
             const cplx cj = ld_c(C + j);
-            if (q < l) pl = term<RC>(coef[e], r.r1[q], cj, pl);
-            else part = term<RC>(coef[e], r.r1[q] * sq[occ<M>(r, q) + 2], cj, part);
+            if (q < l) pl = term<RC>(coef[q + l * M], r.r1[q], cj, pl);
+            else part = term<RC>(coef[q + l * M], r.r1[q] * sq[occ<M>(r, q) + 2], cj, part);
         }
         part.re = fma(r.r1[l], pl.re, part.re);
         part.im = fma(r.r1[l], pl.im, part.im);
+    }
+    return part;
+}
+
+// The same sum with the targets read from a packed slot: group g (entries
+// t = 4g .. 4g+3, t = l(l+1)/2 + q) of this lane at slot + g * nact4.
+template <int M, bool RC>
+__device__ inline cplx chunk_packed(const int32_t *__restrict__ slot, int nact4, const cplx *__restrict__ C,
+                                    const RowState<M> &r, const double *sq, const cplx *coef)
+{
+    cplx part = cmake(0, 0), pl = cmake(0, 0);
+    int32_t grp[4] = {0, 0, 0, 0};
+#pragma unroll
+    for (int l = 0; l < M; l++) {
+#pragma unroll
+        for (int q = 0; q <= l; q++) {
+            const int t = l * (l + 1) / 2 + q;
+            if ((t & 3) == 0) {
+                const Piece4 pc = *reinterpret_cast<const Piece4 *>(slot + (int64_t)(t >> 2) * nact4);
+#pragma unroll
+                for (int u = 0; u < 4; u++) grp[u] = pc.x[u];
+            }
+            const cplx cj = ld_c(C + (int64_t)grp[t & 3]);
+            if (q < l) pl = term<RC>(coef[q + l * M], r.r1[q], cj, pl);
+            else part = term<RC>(coef[q + l * M], r.r1[q] * sq[occ<M>(r, q) + 2], cj, part);
+        }
+        part.re = fma(r.r1[l], pl.re, part.re);
+        part.im = fma(r.r1[l], pl.im, part.im);
+        pl = cmake(0, 0);
     }
     return part;
 }
@@ -209,7 +230,7 @@ __device__ inline cplx chunk_all(const int32_t *__restrict__ chunk, const uint32
 // q and l are runtime values here, so occupations come from the packed copy,
 // square roots from the LDS table and prefix differences from the LDS copy
 template <int M, bool DMAP, bool RC>
-__device__ inline cplx chunk_list(const int32_t *__restrict__ chunk, const uint32_t *Dl, int lane, int T,
+__device__ inline cplx chunk_list(const int32_t *__restrict__ slot, int nact4, const uint32_t *Dl, int lane, int T,
                                   uint32_t base, const cplx *__restrict__ C, const RowState<M> &r,
                                   const double *sq, const int32_t *ents, const cplx *coefs, int n)
 {
@@ -223,7 +244,10 @@ __device__ inline cplx chunk_list(const int32_t *__restrict__ chunk, const uint3
             const bool ok = t + u < n;  // uniform
             const int e = ok ? ents[t + u] : 0;
             const int q = e % M, l = e / M;
-            if (DMAP) j[u] = chunk[e];
+            if (DMAP) {
+                const int t = l * (l + 1) / 2 + q;   // q <= l for every listed entry
+                j[u] = slot[(int64_t)(t >> 2) * nact4 + (t & 3)];
+            }
             else j[u] = (int64_t)(uint32_t)(base - dl<M>(Dl, lane, T, 0, q) - dl<M>(Dl, lane, T, 1, l));
             const int vq = occ<M>(r, q), vl = occ<M>(r, l);
             x[u] = ok ? ((q == l) ? sq[vq + 1] * sq[vq + 2] : sq[vq + 1] * sq[vl + 1]) : 0.0;
@@ -339,16 +363,23 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(SKIP ? 4 : 
         }
     }
 
+    // double jumps: slots of the packed tables (DMAP) follow the jumps
+    // active in any lane, in the order visited below
+    constexpr int E = packed_entries(M);
+    const int lw = threadIdx.x & 63;
+    const uint64_t below = lw ? (~0ull >> (64 - lw)) : 0ull;
+    const int64_t wv = i >> 6;
     // same-orbital double jumps: rules 3, 6
     {
-        const int32_t *sect = DMAP ? b.MapOT + b.strideOT[i] : nullptr;
-        int cnt = 0;
+        int64_t pb = DMAP ? b.poffOT[wv] : 0;
         for (int k = 0; k < M; k++) {
             const int n_nz = SKIP ? c.ot_off[k + 1] - c.ot_off[k] : 1;
             const int vk = occ<M>(r, k);
             const bool act = vk >= 2;
-            const int32_t *chunk = DMAP ? sect + (int64_t)cnt * MM : nullptr;
-            cnt += act;
+            const uint64_t m = DMAP ? __ballot(act) : 0ull;
+            const int nact = __popcll(m);
+            const int32_t *slot = DMAP ? b.packOT + pb + 4 * __popcll(m & below) : nullptr;
+            pb += (int64_t)nact * E;
             if (SKIP && n_nz == 0) continue;
             if (!act) continue;
             uint32_t D0[M], D1[M], tot = 0;
@@ -357,26 +388,31 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(SKIP ? 4 : 
             const uint32_t base = top - tot;
             const double f = sq[vk] * sq[vk - 1];
             cplx part;
-            if (!SKIP) part = chunk_all<M, DMAP, RC>(chunk, D0, D1, base, C, r, sq, c.ot + (int64_t)k * MM);
-            else
-                part = chunk_list<M, DMAP, RC>(chunk, Dl, lane, T, base, C, r, sq, c.nz_e + c.ot_off[k],
+            if (!SKIP) {
+                if (DMAP) part = chunk_packed<M, RC>(slot, 4 * nact, C, r, sq, c.ot + (int64_t)k * MM);
+                else part = chunk_ranked<M, RC>(D0, D1, base, C, r, sq, c.ot + (int64_t)k * MM);
+            } else {
+                part = chunk_list<M, DMAP, RC>(slot, 4 * nact, Dl, lane, T, base, C, r, sq, c.nz_e + c.ot_off[k],
                                                c.nz_c + c.ot_off[k], n_nz);
+            }
             acc.re = fma(f, part.re, acc.re);
             acc.im = fma(f, part.im, acc.im);
         }
     }
     // two-orbital double jumps: rules 7, 9
     {
-        const int32_t *sect = DMAP ? b.MapTT + b.strideTT[i] : nullptr;
-        int cnt = 0, p = 0;
+        int64_t pb = DMAP ? b.poffTT[wv] : 0;
+        int p = 0;
         for (int k = 0; k < M; k++) {
             const int vk = occ<M>(r, k);
             for (int s = k + 1; s < M; s++, p++) {
                 const int n_nz = SKIP ? c.tt_off[p + 1] - c.tt_off[p] : 1;
                 const int vs = occ<M>(r, s);
                 const bool act = vk >= 1 && vs >= 1;
-                const int32_t *chunk = DMAP ? sect + (int64_t)cnt * MM : nullptr;
-                cnt += act;
+                const uint64_t m = DMAP ? __ballot(act) : 0ull;
+                const int nact = __popcll(m);
+                const int32_t *slot = DMAP ? b.packTT + pb + 4 * __popcll(m & below) : nullptr;
+                pb += (int64_t)nact * E;
                 if (SKIP && n_nz == 0) continue;
                 if (!act) continue;
                 uint32_t D0[M], D1[M], tot = 0;
@@ -385,10 +421,13 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(SKIP ? 4 : 
                 const uint32_t base = top - tot;
                 const double f = sq[vk] * sq[vs];
                 cplx part;
-                if (!SKIP) part = chunk_all<M, DMAP, RC>(chunk, D0, D1, base, C, r, sq, c.tt + (int64_t)p * MM);
-                else
-                    part = chunk_list<M, DMAP, RC>(chunk, Dl, lane, T, base, C, r, sq, c.nz_e + c.tt_off[p],
-                                                   c.nz_c + c.tt_off[p], n_nz);
+                if (!SKIP) {
+                    if (DMAP) part = chunk_packed<M, RC>(slot, 4 * nact, C, r, sq, c.tt + (int64_t)p * MM);
+                    else part = chunk_ranked<M, RC>(D0, D1, base, C, r, sq, c.tt + (int64_t)p * MM);
+                } else {
+                    part = chunk_list<M, DMAP, RC>(slot, 4 * nact, Dl, lane, T, base, C, r, sq,
+                                                   c.nz_e + c.tt_off[p], c.nz_c + c.tt_off[p], n_nz);
+                }
                 acc.re = fma(f, part.re, acc.re);
                 acc.im = fma(f, part.im, acc.im);
             }
@@ -412,7 +451,7 @@ inline mcpi_status launch_fast_m(const BasisDev &b, int maps, const HamCoefDev &
     size_t sm = sizeof(double) * (b.N + 3) + sizeof(uint32_t) * M * (b.N + 1) + 16;
     if (skip) sm += sizeof(uint32_t) * 2 * M * 256;
     const bool map1 = maps >= MCPI_MAPS_SINGLE;
-    const bool dmap = maps == MCPI_MAPS_FULL && !rank_doubles;
+    const bool dmap = maps == MCPI_MAPS_FULL && !rank_doubles && b.packTT && b.packOT;
     const bool real = c.real;
 #define MCPI_L(A, B, S, R)                                                                                  \
     {                                                                                                       \

@@ -280,6 +280,96 @@ __global__ void k_double_map(mcpi::BasisDev b)
     }
 }
 
+// ---- wave-packed double-jump tables --
+//
+// The H.v kernel runs one thread per row with lanes = 64 consecutive rows and
+// visits the jumps (k, s) of a chunk kind in a fixed order, uniformly across
+// the wavefront.  In the reference layout every lane streams its own row's
+// chunk, so each 16-byte load of a wavefront touches 64 different cache
+// lines.  The packed layout stores, for wavefront w (rows 64w .. 64w+63) and
+// each jump active in at least one lane (ballot != 0), one slot
+//     slot[g][a][u] = target of entry t = 4g + u for the a-th active lane
+// (t = l(l+1)/2 + q over q <= l, the half of the chunk H.v reads; entries
+// t >= M(M+1)/2 hold the row's own index).  A 16-byte load of all active
+// lanes is then one contiguous run.  Slots follow the reference's chunk
+// order (pairs k < s lexicographic; OT: k ascending), so a lane's slot
+// sequence is its row's chunk sequence with the rows of the wavefront
+// interleaved.
+
+// ints per wavefront (counts fit int32: at most C(32,2) * 64 * 528)
+__global__ void k_packed_counts(int M, int64_t nc, const int32_t *__restrict__ IF, int32_t *__restrict__ cnt_ot,
+                                int32_t *__restrict__ cnt_tt)
+{
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const int64_t w = i >> 6;
+    const int64_t nw = (nc + 63) >> 6;
+    const bool valid = i < nc;
+    uint32_t m1 = 0, m2 = 0;
+    if (valid)
+        for (int j = 0; j < M; j++) {
+            const int x = IF[i * M + j];
+            m1 |= (uint32_t)(x >= 1) << j;
+            m2 |= (uint32_t)(x >= 2) << j;
+        }
+    int ot = 0, tt = 0;
+    for (int k = 0; k < M; k++) {
+        ot += __popcll(__ballot(valid && ((m2 >> k) & 1u)));
+        for (int s = k + 1; s < M; s++) tt += __popcll(__ballot(valid && ((m1 >> k) & 1u) && ((m1 >> s) & 1u)));
+    }
+    if ((threadIdx.x & 63) == 0 && w < nw) {
+        cnt_ot[w] = ot * packed_entries(M);
+        cnt_tt[w] = tt * packed_entries(M);
+    }
+}
+
+template <bool TT>
+__global__ void k_packed_fill(mcpi::BasisDev b)
+{
+    extern __shared__ int smem[];
+    const int M = b.M, N = b.N;
+    int32_t *Bt = smem;                                                 // M*(N+1)
+    int *P = smem + M * (N + 1) + threadIdx.x * (MCPI_MAXM + 1);        // prefix counts
+    for (int t = threadIdx.x; t < M * (N + 1); t += blockDim.x) Bt[t] = b.B[t];
+    __syncthreads();
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const int64_t w = i >> 6;
+    const int64_t nw = (b.nc + 63) >> 6;
+    if (w >= nw) return;   // whole wavefronts past the last row
+    const bool valid = i < b.nc;
+    const int lane = threadIdx.x & 63;
+    const uint64_t below = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
+    uint32_t m1 = 0, m2 = 0;
+    int acc = 0;
+    for (int j = 0; j < M; j++) {
+        const int x = valid ? b.IF[i * M + j] : 0;
+        P[j] = acc;
+        acc += x;
+        m1 |= (uint32_t)(x >= 1) << j;
+        m2 |= (uint32_t)(x >= 2) << j;
+    }
+    const int E = packed_entries(M), MH = M * (M + 1) / 2;
+    int32_t *pack = TT ? b.packTT : b.packOT;
+    int64_t base = (TT ? b.poffTT : b.poffOT)[w];
+    for (int k = 0; k < M; k++) {
+        for (int s = TT ? k + 1 : k; s < (TT ? M : k + 1); s++) {
+            const bool act = valid && (TT ? (((m1 >> k) & 1u) && ((m1 >> s) & 1u)) : ((m2 >> k) & 1u));
+            const uint64_t m = __ballot(act);
+            if (m == 0) continue;
+            const int nact = __popcll(m), my = __popcll(m & below);
+            if (act) {
+                int t = 0;
+                for (int l = 0; l < M; l++)
+                    for (int q = 0; q <= l; q++, t++)
+                        pack[base + ((int64_t)(t >> 2) * nact + my) * 4 + (t & 3)] =
+                            (int32_t)rank_moved(P, Bt, N, M, b.nc, k, s, l, q);
+                for (; t < E; t++) pack[base + ((int64_t)(t >> 2) * nact + my) * 4 + (t & 3)] = (int32_t)i;
+            }
+            (void)MH;
+            base += (int64_t)nact * E;
+        }
+    }
+}
+
 }  // namespace mcpi
 
 using namespace mcpi;
@@ -324,8 +414,8 @@ static void host_tables(int N, int M, std::vector<int32_t> &NCmat, std::vector<i
 static void free_basis(mcpi_basis b)
 {
     if (!b) return;
-    void *ptrs[] = {b->d.IF, b->d.Map, b->d.MapOT, b->d.MapTT, b->d.strideOT, b->d.strideTT,
-                    b->d.B, b->d.NCmat};
+    void *ptrs[] = {b->d.IF,     b->d.Map,    b->d.MapOT,  b->d.MapTT,  b->d.strideOT, b->d.strideTT,
+                    b->d.B,      b->d.NCmat,  b->d.packOT, b->d.packTT, b->d.poffOT,   b->d.poffTT};
     for (void *p : ptrs)
         if (p) (void)hipFree(p);
     delete b;
@@ -394,6 +484,33 @@ static mcpi_status basis_init(int N, int M, const int32_t *IF_host, const int32_
     return MCPI_OK;
 }
 
+// wave-packed double-jump tables (built from IF, independent of MapOT/MapTT)
+static mcpi_status build_packed(mcpi_basis b)
+{
+    const int M = b->d.M, N = b->d.N;
+    const int64_t nc = b->d.nc, nw = (nc + 63) / 64;
+    if (b->d.packTT) return MCPI_OK;
+    int32_t *cnt_ot = nullptr, *cnt_tt = nullptr;
+    MCPI_HIP(hipMalloc(&cnt_ot, nw * 4));
+    MCPI_HIP(hipMalloc(&cnt_tt, nw * 4));
+    const int64_t grid = (nw * 64 + 255) / 256;
+    hipLaunchKernelGGL(k_packed_counts, dim3(grid), dim3(256), 0, 0, M, nc, b->d.IF, cnt_ot, cnt_tt);
+    MCPI_HIP(hipGetLastError());
+    MCPI_CHECK(alloc_dev((void **)&b->d.poffOT, (nw + 1) * 8, &b->device_bytes));
+    MCPI_CHECK(alloc_dev((void **)&b->d.poffTT, (nw + 1) * 8, &b->device_bytes));
+    MCPI_CHECK(scan_to_strides(nw, cnt_ot, 1, b->d.poffOT, &b->pot_len));
+    MCPI_CHECK(scan_to_strides(nw, cnt_tt, 1, b->d.poffTT, &b->ptt_len));
+    MCPI_CHECK(alloc_dev((void **)&b->d.packOT, b->pot_len * 4, &b->device_bytes));
+    MCPI_CHECK(alloc_dev((void **)&b->d.packTT, b->ptt_len * 4, &b->device_bytes));
+    const size_t sm = sizeof(int) * (M * (N + 1) + 256 * (MCPI_MAXM + 1));
+    if (b->pot_len > 0) hipLaunchKernelGGL(k_packed_fill<false>, dim3(grid), dim3(256), sm, 0, b->d);
+    if (b->ptt_len > 0) hipLaunchKernelGGL(k_packed_fill<true>, dim3(grid), dim3(256), sm, 0, b->d);
+    MCPI_HIP(hipGetLastError());
+    MCPI_HIP(hipFree(cnt_ot));
+    MCPI_HIP(hipFree(cnt_tt));
+    return MCPI_OK;
+}
+
 static mcpi_status build_maps_parts(mcpi_basis b, bool want_map, bool want_ot, bool want_tt)
 {
     const int M = b->d.M, N = b->d.N;
@@ -438,6 +555,7 @@ static mcpi_status build_maps_parts(mcpi_basis b, bool want_map, bool want_ot, b
     if (b->d.Map && b->d.MapOT && b->d.MapTT) b->maps = MCPI_MAPS_FULL;
     else if (b->d.Map) b->maps = MCPI_MAPS_SINGLE;
     else b->maps = MCPI_MAPS_NONE;
+    if (b->maps == MCPI_MAPS_FULL) MCPI_CHECK(build_packed(b));
     MCPI_HIP(hipDeviceSynchronize());
     return MCPI_OK;
 }
@@ -491,6 +609,10 @@ mcpi_status basis_upload_raw(int N, int M, const int32_t *IF, const int32_t *NCm
             if (!st && tt_len && hipMemcpy(b->d.MapTT, MapTT, tt_len * 4, hipMemcpyHostToDevice) != hipSuccess)
                 st = MCPI_ERR_HIP;
         }
+    }
+    if (!st && b->maps == MCPI_MAPS_FULL) {
+        st = build_packed(b);
+        if (!st && hipDeviceSynchronize() != hipSuccess) st = MCPI_ERR_HIP;
     }
     if (st) {
         if (st == MCPI_ERR_HIP) set_error("upload of reference tables failed");
@@ -571,6 +693,8 @@ mcpi_status mcpi_hip_basis_info(mcpi_basis b, mcpi_basis_info *info)
     info->ot_len = b->ot_len;
     info->tt_len = b->tt_len;
     info->device_bytes = b->device_bytes;
+    info->pot_len = b->pot_len;
+    info->ptt_len = b->ptt_len;
     return MCPI_OK;
 }
 
@@ -631,7 +755,11 @@ mcpi_status mcpi_hip_basis_copy_range(mcpi_basis b, int which, int64_t offset, i
     case 3: src = b->d.IF; len = b->d.nc * b->d.M; break;
     case 4: src = b->d.strideOT; len = b->d.strideOT ? b->d.nc + 1 : 0; esize = 8; break;
     case 5: src = b->d.strideTT; len = b->d.strideTT ? b->d.nc + 1 : 0; esize = 8; break;
-    default: MCPI_ARG(false, "which must be 0..5");
+    case 6: src = b->d.packOT; len = b->pot_len; break;
+    case 7: src = b->d.packTT; len = b->ptt_len; break;
+    case 8: src = b->d.poffOT; len = b->d.poffOT ? (b->d.nc + 63) / 64 + 1 : 0; esize = 8; break;
+    case 9: src = b->d.poffTT; len = b->d.poffTT ? (b->d.nc + 63) / 64 + 1 : 0; esize = 8; break;
+    default: MCPI_ARG(false, "which must be 0..9");
     }
     MCPI_ARG(src, "table not stored in this basis");
     MCPI_ARG(offset + count <= len, "range beyond the table");

@@ -694,16 +694,17 @@ static mcpi_status lanczos_core(int K, const mcpi_ham *hams, const cplx *const *
     for (HvGroup &g : groups) {
         const mcpi_ham h0 = hams[g.members[0]];
         bool ok = g.members.size() > 1 && has_fast_kernel(g.basis->d.N, g.basis->d.M);
-        bool real = true;
+        bool real = true, dense = true;
         for (int x : g.members) {
             const mcpi_ham h = hams[x];
             ok = ok && h->skip_zero == h0->skip_zero && h->schedule == h0->schedule && h->schedule != 1 &&
                  h->coef_bytes == h0->coef_bytes;
             real = real && h->c.real;
+            dense = dense && h->c.dense;
         }
         g.batched = ok;
         if (ok) {
-            g.skip = h0->skip_zero;
+            g.skip = h0->skip_zero && !dense;
             g.cbytes = h0->coef_bytes;
             g.c0.real = real ? 1 : 0;
             for (size_t k = 0; k < g.members.size(); k++) batched_member[g.members[k]] = (int)k;

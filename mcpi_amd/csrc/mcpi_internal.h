@@ -92,7 +92,16 @@ struct BasisDev {
     int64_t *strideTT;  // nc+1 entries (last = |MapTT|) or null
     int32_t *B;         // (M) x (N+1) table B[j*(N+1) + p], row 0 unused
     int32_t *NCmat;     // (N+1)*(M+1), reference layout
+    // Wave-packed double-jump targets (full maps only; see basis.hip): for
+    // each wavefront of 64 consecutive rows w, the slots of the jumps active
+    // in any of its lanes, from int offset poffOT[w] / poffTT[w]
+    int32_t *packOT, *packTT;
+    int64_t *poffOT, *poffTT;   // nw + 1 entries, nw = ceil(nc / 64)
 };
+
+// entries of a packed slot: the M(M+1)/2 targets q <= l of a chunk, in
+// triangular order t = l(l+1)/2 + q, padded to a multiple of four
+__host__ __device__ constexpr int packed_entries(int M) { return (M * (M + 1) / 2 + 3) / 4 * 4; }
 
 }  // namespace mcpi
 
@@ -101,6 +110,7 @@ struct mcpi_basis_s {
     mcpi::BasisDev d;
     int maps;
     int64_t ot_len, tt_len, map_len;
+    int64_t pot_len, ptt_len;   // ints in packOT / packTT
     int64_t device_bytes;
 };
 
@@ -130,6 +140,8 @@ struct HamCoefDev {
     int32_t *nz_e;   // entry index q + l*M of each listed term
     cplx *nz_c;      // its folded coefficient
     int real;        // 1 if every coefficient above has a zero imaginary part
+    int dense;       // 1 if at least half of the double-jump terms are non-zero:
+                     // walking the lists would cost more than it skips
 };
 
 // the same tables d bytes further on (instance x of a batch, or a copy of

@@ -11,6 +11,11 @@
 // The host expands the representative terms into full matrices with the
 // reference's symmetry copies.
 //
+// Targets come from the stored maps when the basis has them and are ranked
+// on the fly otherwise (closed-form FockToIndex over the lane's prefix
+// counts, configurationsMap.h:273-305), so the adjoint also runs on bases
+// that store only Map, or no map at all.
+//
 // Term classes (compact layouts, P = M(M-1)/2 pairs, pq = pair index):
 //   class 0 (single jumps and diagonal):
 //     r1d[k] | r1o[kl] | R1[k] | R2[ks] | R4[kl] | R5[ks] | R7[kl][s]
@@ -51,6 +56,21 @@ __device__ inline void wave_add(cplx z, cplx *acc, int idx, bool any)
 
 __device__ inline bool wave_any(bool p) { return __ballot(p) != 0; }
 
+// rank of this lane's row with particles removed from a, b2 (b2 = -1: none)
+// and added to c, d (d = -1: none); occupations from the LDS tile
+__device__ inline int64_t rank_lane(const uint8_t *Vs, int t, int T, const int32_t *Bt, int N, int M, int64_t nc,
+                                    int a, int b2, int c, int d)
+{
+    int P = 0;
+    int64_t s = 0;
+    for (int j = 1; j < M; j++) {
+        P += Vs[(j - 1) * T + t];
+        const int p = P - (a < j) - (b2 >= 0 && b2 < j) + (c < j) + (d >= 0 && d < j);
+        s += Bt[j * (N + 1) + p];
+    }
+    return nc - 1 - s;
+}
+
 template <int CLASS>
 __global__ __launch_bounds__(256) void k_rho(BasisDev b, const cplx *__restrict__ C, int p0, int p1,
                                             int nterms, cplx *__restrict__ partial)
@@ -61,9 +81,11 @@ __global__ __launch_bounds__(256) void k_rho(BasisDev b, const cplx *__restrict_
     const int64_t nc = b.nc, MM = (int64_t)M * M;
     cplx *acc = (cplx *)smem_r;                       // nterms
     double *sq = (double *)(acc + nterms);            // N+3
-    uint8_t *Vs = (uint8_t *)(sq + N + 3);            // M*T
+    int32_t *Bt = (int32_t *)(sq + N + 3);            // M*(N+1) rank table
+    uint8_t *Vs = (uint8_t *)(Bt + M * (N + 1));      // M*T
     for (int x = t; x < nterms; x += T) acc[x] = cmake(0, 0);
     for (int x = t; x < N + 3; x += T) sq[x] = sqrt((double)x);
+    for (int x = t; x < M * (N + 1); x += T) Bt[x] = b.B[x];
     __syncthreads();
 
     for (int64_t tile = (int64_t)blockIdx.x * T; tile < nc; tile += (int64_t)gridDim.x * T) {
@@ -95,7 +117,8 @@ __global__ __launch_bounds__(256) void k_rho(BasisDev b, const cplx *__restrict_
                     if (!wave_any(act)) continue;
                     cplx g = cmake(0, 0);
                     if (act) {
-                        int64_t j = b.Map[i + (int64_t)(k + M * l) * nc];
+                        const int64_t j = b.Map ? (int64_t)b.Map[i + (int64_t)(k + M * l) * nc]
+                                                : rank_lane(Vs, t, T, Bt, N, M, nc, k, -1, l, -1);
                         g = cfma_conj(ci, ldcr(C + j), cmake(0, 0));   // conj(C_i) C_j
                     }
                     const double root = act ? sq[vk] * sq[vl + 1] : 0.0;
@@ -117,14 +140,15 @@ __global__ __launch_bounds__(256) void k_rho(BasisDev b, const cplx *__restrict_
             for (int k = 0; k < M; k++) {
                 const int vk = V(k);
                 const bool act = valid && vk >= 2;
-                const int64_t base = act ? b.strideOT[i] + (int64_t)cot * MM : 0;
+                const int64_t base = (act && b.MapOT) ? b.strideOT[i] + (int64_t)cot * MM : 0;
                 if (act) cot++;
                 if (!wave_any(act)) continue;
                 const double fk = act ? sq[vk] * sq[vk - 1] : 0.0;
                 for (int q = k + 1; q < M; q++) {
                     cplx z = cmake(0, 0);
                     if (act) {
-                        int64_t j = b.MapOT[base + q + q * M];
+                        const int64_t j = b.MapOT ? (int64_t)b.MapOT[base + q + q * M]
+                                                  : rank_lane(Vs, t, T, Bt, N, M, nc, k, k, q, q);
                         z = cscale(cfma_conj(ci, ldcr(C + j), cmake(0, 0)),
                                    fk * sq[V(q) + 1] * sq[V(q) + 2]);
                     }
@@ -136,7 +160,8 @@ __global__ __launch_bounds__(256) void k_rho(BasisDev b, const cplx *__restrict_
                         if (q == k || l == k) continue;
                         cplx z = cmake(0, 0);
                         if (act) {
-                            int64_t j = b.MapOT[base + l + q * M];
+                            const int64_t j = b.MapOT ? (int64_t)b.MapOT[base + l + q * M]
+                                                      : rank_lane(Vs, t, T, Bt, N, M, nc, k, k, q, l);
                             z = cscale(cfma_conj(ci, ldcr(C + j), cmake(0, 0)),
                                        fk * sq[V(q) + 1] * sq[V(l) + 1]);
                         }
@@ -150,7 +175,7 @@ __global__ __launch_bounds__(256) void k_rho(BasisDev b, const cplx *__restrict_
                 for (int s = k + 1; s < M; s++, p++) {
                     const int vk = V(k), vs = V(s);
                     const bool act = valid && vk >= 1 && vs >= 1;
-                    const int64_t base = act ? b.strideTT[i] + (int64_t)ctt * MM : 0;
+                    const int64_t base = (act && b.MapTT) ? b.strideTT[i] + (int64_t)ctt * MM : 0;
                     if (act) ctt++;
                     if (p < p0 || p >= p1) continue;
                     if (!wave_any(act)) continue;
@@ -161,7 +186,8 @@ __global__ __launch_bounds__(256) void k_rho(BasisDev b, const cplx *__restrict_
                             if (q == k || q == s || l == k || l == s) continue;
                             cplx z = cmake(0, 0);
                             if (act) {
-                                int64_t j = b.MapTT[base + q + l * M];
+                                const int64_t j = b.MapTT ? (int64_t)b.MapTT[base + q + l * M]
+                                                          : rank_lane(Vs, t, T, Bt, N, M, nc, k, s, q, l);
                                 z = cscale(cfma_conj(ci, ldcr(C + j), cmake(0, 0)),
                                            fks * sq[V(q) + 1] * sq[V(l) + 1]);
                             }
@@ -199,7 +225,8 @@ static mcpi_status run_class(mcpi_basis bs, int cls, const cplx *C, int p0, int 
     MCPI_HIP(hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev));
     int64_t tiles = (b.nc + T - 1) / T;
     int nblocks = (int)std::min<int64_t>(tiles, 2 * (int64_t)ncu);
-    size_t sm = sizeof(cplx) * nterms + sizeof(double) * (b.N + 3) + (size_t)b.M * T + 16;
+    size_t sm = sizeof(cplx) * nterms + sizeof(double) * (b.N + 3) + sizeof(int32_t) * b.M * (b.N + 1) +
+                (size_t)b.M * T + 16;
     size_t max_sm = 160 * 1024;
     MCPI_ARG(sm <= max_sm, "density-matrix accumulators exceed LDS");
     cplx *partial = nullptr, *out = nullptr;
@@ -236,7 +263,6 @@ static mcpi_status densities(mcpi_basis bs, const cplx *C, cplx *rho1, cplx *rho
 {
     const int M = bs->d.M;
     const int P = M * (M - 1) / 2;
-    MCPI_ARG(bs->d.Map, "density matrices need the single-jump map");
     std::vector<cplx> a;
     const int nA = M + P + M + P + P + P + P * M;
     MCPI_CHECK(run_class(bs, 0, C, 0, 0, nA, a));
@@ -253,7 +279,6 @@ static mcpi_status densities(mcpi_basis bs, const cplx *C, cplx *rho1, cplx *rho
         }
     }
     if (!rho2) return MCPI_OK;
-    MCPI_ARG(bs->d.MapOT && bs->d.MapTT, "two-body density needs the double-jump maps");
     std::vector<cplx> ob, tt((size_t)P * P);
     MCPI_CHECK(run_class(bs, 1, C, 0, 0, P + M * P, ob));
     // class 2 in pair ranges that fit in LDS (about 120 KB of accumulators)

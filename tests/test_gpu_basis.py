@@ -10,6 +10,7 @@ import pytest
 import mcpi_amd
 import mcpi_amd.reference as R
 import oracle
+from tests import packed
 from tests.gpu_util import note
 
 pytestmark = pytest.mark.gpu
@@ -35,6 +36,24 @@ def test_tables_bit_exact(oracle_lib, N, M):
         assert np.array_equal(h.IF(), o.IF)
         if maps == "single":
             assert np.array_equal(h.Map(), o.Map)
+
+
+@pytest.mark.parametrize("N,M", [(2, 3), (5, 5), (4, 6), (6, 4), (3, 7), (7, 3), (6, 9), (1, 4), (0, 3), (1, 1)])
+def test_packed_double_jump_tables(oracle_lib, N, M):
+    """Wave-packed OT/TT tables equal the oracle's reference-layout maps
+    re-laid out (tests/packed.py), every entry and every offset."""
+    note(f"packed tables N={N} M={M}")
+    g = mcpi_amd.Basis(N, M, "full")
+    o = oracle.Basis(N, M)
+    nw = (g.nc + 63) // 64
+    for tt, (Mp, st) in ((False, (o.MapOT, o.strideOT)), (True, (o.MapTT, o.strideTT))):
+        pack, poff = packed.expected_packed(o.IF.reshape(-1, M), M, Mp, st, tt)
+        name = "TT" if tt else "OT"
+        got_off = g.copy_range("poff" + name, 0, nw + 1)
+        assert np.array_equal(got_off, poff), name
+        n = g.ptt_len if tt else g.pot_len
+        assert n == pack.size
+        assert np.array_equal(g.copy_range("pack" + name, 0, n), pack), name
 
 
 def test_golden_tables(oracle_lib):
@@ -118,6 +137,26 @@ def test_full_maps_n12_m12(oracle_lib):
     rows = np.concatenate([[0, 1, g.nc - 2, g.nc - 1], rng.integers(0, g.nc, 40)])
     note("full maps N=12 M=12 sampled brute force")
     _sample_rows_check(g, N, M, rows)
+    # wave-packed tables: sampled wavefronts re-laid out from the reference
+    # layout maps (pinned above), and the totals
+    note("packed tables N=12 M=12 sampled wavefronts")
+    nw = (g.nc + 63) // 64
+    for name, tt, n_dbl in (("OT", False, M * nc2), ("TT", True, math.comb(M, 2) * nc2)):
+        poff = g.copy_range("poff" + name, 0, nw + 1)
+        assert poff[0] == 0 and poff[-1] == (g.ptt_len if tt else g.pot_len)
+        assert np.all(np.diff(poff) >= 0)
+        assert poff[-1] == n_dbl * packed.entries(M)   # one slot row per reference chunk
+        for w in np.concatenate([[0, nw - 1], rng.integers(0, nw, 12)]):
+            r0, r1 = 64 * int(w), min(64 * int(w) + 64, g.nc)
+            rows_w = list(range(r0, r1))
+            IFw = g.copy_range("IF", r0 * M, (r1 - r0) * M).reshape(-1, M)
+            IFfull = np.zeros((r1, M), np.int32)
+            IFfull[r0:r1] = IFw
+            st = g.copy_range("stride" + name, r0, r1 - r0 + 1)
+            Mp = g.copy_range("Map" + name, int(st[0]), int(st[-1] - st[0]))
+            exp = packed.expected_wave(IFfull, M, rows_w, Mp, st - st[0], r0, tt)
+            got = g.copy_range("pack" + name, int(poff[w]), int(poff[w + 1] - poff[w]))
+            assert np.array_equal(got, exp), (name, w)
 
 
 def test_single_map_n16_m12(oracle_lib):

@@ -13,11 +13,14 @@ pytestmark = pytest.mark.gpu
 
 
 @pytest.mark.parametrize("N,M", [(1, 3), (2, 3), (5, 5), (4, 6), (6, 4), (3, 7), (7, 9)])
-def test_rho_matches_oracle(oracle_lib, N, M):
-    note(f"rho N={N} M={M}")
+@pytest.mark.parametrize("maps", ["full", "single", "none"])
+def test_rho_matches_oracle(oracle_lib, N, M, maps):
+    """Targets read from the stored maps (full), or ranked on the fly for the
+    double jumps (single) or for every jump (none)."""
+    note(f"rho N={N} M={M} maps={maps}")
     o = oracle.Basis(N, M)
     C, _, _ = oracle.random_inputs(N, M, seed=21 + N)
-    b = mcpi_amd.Basis(N, M, "full")
+    b = mcpi_amd.Basis(N, M, maps)
     dC = mcpi_amd.DeviceArray.from_numpy(C)
     assert_close(b.rho1(dC), o.rho1(C), 1e-12, "rho1")
     assert_close(b.rho2(dC), o.rho2(C), 1e-12, "rho2")
@@ -85,3 +88,25 @@ def test_vjp_finite_differences(oracle_lib):
         fd = (e0(Ho + h * Do, Hint + h * Dint)[0] - e0(Ho - h * Do, Hint - h * Dint)[0]) / (2 * h)
         an = np.real(np.sum(gHo * Do) + np.sum(gHint * Dint))
         assert abs(fd - an) <= 1e-5 * max(1.0, abs(an)), (t, fd, an)
+
+
+def test_rho_single_map_n16_m12_invariants(oracle_lib):
+    """N=16, M=12 with only Map stored (nc 13,037,895): rho2's double jumps
+    ranked on the fly; partial traces, contraction, Hermiticity, and the
+    energy identity against H.v of the same basis."""
+    N, M = 16, 12
+    note("rho invariants N=16 M=12 single map")
+    b = mcpi_amd.Basis(N, M, "single")
+    rng = np.random.default_rng(16)
+    C = rng.standard_normal(b.nc) + 1j * rng.standard_normal(b.nc)
+    C /= np.linalg.norm(C)
+    _, Ho, Hint = oracle.random_inputs(2, M, seed=5, sigma=0.1)
+    dC = mcpi_amd.DeviceArray.from_numpy(C)
+    r1, r2 = b.rho1(dC), b.rho2(dC)
+    assert abs(np.trace(r1) - N) < 1e-10
+    assert abs(np.einsum("ksks->", r2) - N * (N - 1)) < 1e-8
+    assert np.allclose(np.einsum("kssl->kl", r2), (N - 1) * r1, atol=1e-10)
+    assert np.allclose(r2, np.conj(r2.transpose(3, 2, 1, 0)), atol=1e-12)
+    E = np.vdot(C, mcpi_amd.Hamiltonian(b, Ho, Hint)(C))
+    E2 = np.sum(Ho * r1) + 0.5 * np.sum(Hint * r2)
+    assert abs(E - E2) <= 1e-11 * abs(E)

@@ -332,3 +332,20 @@ def test_bethe_solver(oracle_lib):
     assert abs(bethe.energy_per_particle(N, 1e-4) - 1e-4 * (N - 1) / 2) < 1e-7
     assert abs(bethe.energy_per_particle(N, 1e7) - bethe.tonks_girardeau(N)) < 1e-4
     assert abs(bethe.tonks_girardeau(5) - 4 * np.pi ** 2) < 1e-12
+
+
+@pytest.mark.parametrize("N,M", [(4, 5), (6, 4), (3, 7)])
+def test_packed_layout_restatement(oracle_lib, N, M):
+    """The test-side wave-packed layout holds every reference chunk once:
+    E entries per chunk, offsets increasing, and each slot row decodes to the
+    q <= l half of its chunk."""
+    from tests import packed
+    o = oracle.Basis(N, M)
+    IF = o.IF.reshape(-1, M)
+    for tt, Mp, st in ((False, o.MapOT, o.strideOT), (True, o.MapTT, o.strideTT)):
+        pack, poff = packed.expected_packed(IF, M, Mp, st, tt)
+        assert pack.size == Mp.size // (M * M) * packed.entries(M)
+        assert poff[0] == 0 and poff[-1] == pack.size and np.all(np.diff(poff) >= 0)
+        # row 0's first chunk (if any) sits in wave 0's first slot, lane 0
+        if st.size and (st[1] if st.size > 1 else Mp.size) > st[0]:
+            assert pack[0] == Mp[0]
