@@ -263,6 +263,12 @@ mcpi_status mcpi_hip_lanczos_batched(int n, const mcpi_ham *hams,
                                      const mcpi_cplx *const *C0_dev, int lm,
                                      double *E0, int *steps,
                                      mcpi_cplx *const *C_out_dev);
+/* H.v kernel of batched groups (instances sharing a basis): 0 = chosen by
+ * timing the candidates on the first step (default); 1 = one launch of the
+ * specialised kernel over the group; 2 = the multi-vector kernel (one pass
+ * over the rows for up to four instances that share the zero pattern of
+ * their coefficients), where the group allows it.  Process-wide. */
+mcpi_status mcpi_hip_set_batched_kernel(int kind);
 
 #ifdef __cplusplus
 }

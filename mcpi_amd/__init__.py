@@ -105,6 +105,7 @@ def lib():
         "mcpi_hip_tridiag_lowest": (st, [ct.c_int, _f64p, _f64p, ct.POINTER(ct.c_double), _f64p]),
         "mcpi_hip_lanczos_batched": (st, [ct.c_int, ct.POINTER(_vp), ct.POINTER(_vp), ct.c_int,
                                           _f64p, _i32p, ct.POINTER(_vp)]),
+        "mcpi_hip_set_batched_kernel": (st, [ct.c_int]),
     }
     for name, (res, args) in sig.items():
         f = getattr(L, name)
@@ -404,6 +405,12 @@ def lanczos_batched(hams, starts, lm: int, outs=None):
     steps = np.zeros(max(n, 1), np.int32)
     _check(lib().mcpi_hip_lanczos_batched(n, H, S, lm, E0, steps, O), "lanczos_batched")
     return E0[:n], steps[:n]
+
+
+def set_batched_kernel(kind: int) -> None:
+    """H.v kernel of batched groups: 0 timed (default), 1 batched specialised
+    kernel, 2 multi-vector kernel where allowed (mcpi_hip_set_batched_kernel)."""
+    _check(lib().mcpi_hip_set_batched_kernel(int(kind)), "set_batched_kernel")
 
 
 # ----------------------------------------------------------- models --

@@ -159,6 +159,16 @@ mcpi_status build_coefficients(int M, const cplx *Ho, const cplx *Hint, HamCoefD
     for (const std::vector<cplx> *v : {&dk, &d1, &d2, &A0, &A, &ot, &tt})
         for (const cplx &z : *v) real = real && z.im == 0.0;
     c->real = real ? 1 : 0;
+    uint64_t sig = 1469598103934665603ull;   // FNV-1a over the zero pattern
+    auto mix = [&sig](const void *p, size_t n) {
+        const unsigned char *c = (const unsigned char *)p;
+        for (size_t x = 0; x < n; x++) sig = (sig ^ c[x]) * 1099511628211ull;
+    };
+    mix(m1j.data(), m1j.size() * 4);
+    mix(ot_off.data(), ot_off.size() * 4);
+    mix(tt_off.data(), tt_off.size() * 4);
+    mix(nz_e.data(), (size_t)tt_off[P] * 4);
+    c->zsig = sig;
     const int64_t listed = tt_off[P];   // every listed (non-zero) double-jump term
     const int64_t half = (int64_t)(M + P) * (M * (M + 1) / 2);   // q <= l entries of every chunk kind
     c->dense = (2 * listed >= half) ? 1 : 0;
@@ -359,6 +369,26 @@ mcpi_status launch_apply_batch(const BasisDev &b, int maps, const HamCoefDev &c0
 }
 
 bool has_fast_kernel(int N, int M) { return N <= MCPI_MAXN && M >= 3 && M <= 13; }
+
+#define MCPI_MULTI_ARGS const BasisDev &, const HamCoefDev &, int, int, size_t, const cplx *, cplx *, int64_t, hipStream_t
+mcpi_status launch_multi_a(MCPI_MULTI_ARGS);
+mcpi_status launch_multi_b(MCPI_MULTI_ARGS);
+mcpi_status launch_multi_c(MCPI_MULTI_ARGS);
+#undef MCPI_MULTI_ARGS
+
+// One pass of the multi-vector kernel over nbatch instances that share the
+// basis and the zero pattern (c0.zsig), in the batched layout.  sched 2 reads
+// the packed double-jump tables, 3 ranks the targets.  Needs Map.
+mcpi_status launch_apply_multi(const BasisDev &b, const HamCoefDev &c0, int sched, int nbatch, size_t cstride,
+                               const cplx *C0, cplx *out0, int64_t vstride, hipStream_t st)
+{
+    if (b.N > MCPI_MAXN || !b.Map) return MCPI_ERR_STATE;
+    const int rd = sched == 3;
+    if (b.M >= 3 && b.M <= 6) return launch_multi_a(b, c0, rd, nbatch, cstride, C0, out0, vstride, st);
+    if (b.M >= 7 && b.M <= 10) return launch_multi_b(b, c0, rd, nbatch, cstride, C0, out0, vstride, st);
+    if (b.M >= 11 && b.M <= 13) return launch_multi_c(b, c0, rd, nbatch, cstride, C0, out0, vstride, st);
+    return MCPI_ERR_STATE;
+}
 
 mcpi_status launch_apply_raw(const BasisDev &b, int maps, const HamCoefDev &c, int skip_zero,
                              int schedule, const cplx *C, cplx *out, hipStream_t st)

@@ -263,7 +263,9 @@ __device__ inline cplx chunk_list(const int32_t *__restrict__ slot, int nact4, c
 
 // Occupancy: the zero-skipping kernels fit in 128 VGPRs (4 waves per SIMD);
 // the every-term kernels keep a whole chunk's gathers in flight and are
-// given up to 256 VGPRs (2 waves per SIMD) rather than spill.
+// given up to 256 VGPRs rather than spill (the packed-table variant uses
+// about 150, 3 waves per SIMD; forcing 4 waves spills 16-112 bytes per lane
+// depending on M and gained 3% at N = M = 12).
 // BATCH: nbatch instances share the basis.  Instance x's coefficient tables
 // are instance 0's shifted by x * cstride bytes and its vectors sit
 // x * vstride elements after C0 / out0, so every pointer derives from a

@@ -36,6 +36,7 @@ mcpi_status lanczos_api(mcpi_ham h, int lm, const cplx *C0, cplx *lvec, cplx *di
 mcpi_status tridiag_lowest(int n, const double *d, const double *e, double *lambda, double *z);
 mcpi_status lanczos_batched_api(int K, const mcpi_ham *hams, const cplx *const *C0, int lm, double *E0,
                                 int *steps, cplx *const *Cout, hipStream_t s);
+mcpi_status set_batched_kernel(int kind);
 
 // one library stream (default priority) for calls that do not pass one
 static hipStream_t lib_stream()
@@ -246,6 +247,8 @@ mcpi_status mcpi_hip_tridiag_lowest(int n, const double *d, const double *e, dou
 {
     return tridiag_lowest(n, d, e, lambda, z);
 }
+
+mcpi_status mcpi_hip_set_batched_kernel(int kind) { return set_batched_kernel(kind); }
 
 mcpi_status mcpi_hip_lanczos_batched(int n, const mcpi_ham *hams, const mcpi_cplx *const *C0_dev, int lm,
                                      double *E0, int *steps, mcpi_cplx *const *C_out_dev)

@@ -612,6 +612,8 @@ struct HvGroup {
     mcpi_basis basis = nullptr;
     std::vector<int> members;
     bool batched = false;
+    bool multi_ok = false;        // members share the zero pattern: multi-vector kernel allowed
+    int mode = 1;                 // 1 batched specialised kernel, 2 multi-vector kernel
     int sched = 2, skip = 0;
     size_t cbytes = 0;            // coefficient block size = stride
     HamCoefDev c0{};              // member 0's tables inside `coefs`
@@ -622,6 +624,9 @@ struct HvGroup {
 static mcpi_status launch_group(const HvGroup &g, const mcpi_ham *hams, const std::vector<LzInst> &hin,
                                 hipStream_t s)
 {
+    if (g.batched && g.mode == 2)
+        return launch_apply_multi(g.basis->d, g.c0, g.sched, (int)g.members.size(), g.cbytes, g.vin, g.HC,
+                                  g.basis->d.nc, s);
     if (g.batched)
         return launch_apply_batch(g.basis->d, g.basis->maps, g.c0, g.skip, g.sched, (int)g.members.size(), g.cbytes,
                                   g.vin, g.HC, g.basis->d.nc, s);
@@ -629,23 +634,53 @@ static mcpi_status launch_group(const HvGroup &g, const mcpi_ham *hams, const st
     return MCPI_OK;
 }
 
-// time the specialised schedules on the whole group, keep the faster
-static mcpi_status tune_group(HvGroup &g, const mcpi_ham *hams, const std::vector<LzInst> &hin, hipStream_t s)
+// Time the candidate (kernel, schedule) pairs on the whole group and keep
+// the fastest: the batched specialised kernel with the double-jump targets
+// read (2) or ranked (3), and, when the members share their zero pattern,
+// the multi-vector kernel with the same two schedules.  fixed: a schedule
+// the caller set on the Hamiltonians (0: free).
+// kernel choice for batched groups: 0 = timed, 1 = batched specialised
+// kernel only, 2 = multi-vector kernel where the group allows it
+static int g_batched_kernel = 0;
+
+mcpi_status set_batched_kernel(int kind)
 {
-    std::vector<int> cand{2};
-    if (g.basis->maps == MCPI_MAPS_FULL) cand.push_back(3);
+    MCPI_ARG(kind >= 0 && kind <= 2, "kind must be 0, 1 or 2");
+    g_batched_kernel = kind;
+    return MCPI_OK;
+}
+
+static mcpi_status tune_group(HvGroup &g, const mcpi_ham *hams, const std::vector<LzInst> &hin, int fixed,
+                              hipStream_t s)
+{
+    std::vector<std::pair<int, int>> cand;   // (mode, sched)
+    const bool full = g.basis->maps == MCPI_MAPS_FULL;
+    for (int mode = 1; mode <= 2; mode++) {
+        if (mode == 2 && !g.multi_ok) continue;
+        if (g_batched_kernel == 1 && mode == 2) continue;
+        if (g_batched_kernel == 2 && mode == 1 && g.multi_ok) continue;
+        for (int sc = 2; sc <= 3; sc++) {
+            if (fixed && sc != fixed) continue;
+            if (sc == 3 && !full && mode == 1) continue;   // without stored doubles 2 already ranks
+            if (sc == 2 && !full && mode == 2) continue;
+            cand.push_back({mode, sc});
+        }
+    }
+    if (cand.empty()) cand.push_back({1, fixed ? fixed : 2});
     if (cand.size() == 1) {
-        g.sched = cand[0];
+        g.mode = cand[0].first;
+        g.sched = cand[0].second;
         return MCPI_OK;
     }
     hipEvent_t e0, e1;
     MCPI_HIP(hipEventCreate(&e0));
     MCPI_HIP(hipEventCreate(&e1));
     float best = 0;
-    int pick = cand[0];
+    std::pair<int, int> pick = cand[0];
     mcpi_status rc = MCPI_OK;
-    for (int c : cand) {
-        g.sched = c;
+    for (size_t x = 0; x < cand.size(); x++) {
+        g.mode = cand[x].first;
+        g.sched = cand[x].second;
         rc = launch_group(g, hams, hin, s);   // warm-up
         if (rc != MCPI_OK) break;
         if (hipEventRecord(e0, s) != hipSuccess) { rc = MCPI_ERR_HIP; break; }
@@ -656,12 +691,13 @@ static mcpi_status tune_group(HvGroup &g, const mcpi_ham *hams, const std::vecto
         }
         float ms = 0;
         (void)hipEventElapsedTime(&ms, e0, e1);
-        if (c == cand[0] || ms < best) { best = ms; pick = c; }
+        if (x == 0 || ms < best) { best = ms; pick = cand[x]; }
     }
     (void)hipEventDestroy(e0);
     (void)hipEventDestroy(e1);
     if (rc == MCPI_ERR_HIP) set_error("H.v schedule timing failed");
-    g.sched = pick;
+    g.mode = pick.first;
+    g.sched = pick.second;
     return rc;
 }
 
@@ -694,17 +730,19 @@ static mcpi_status lanczos_core(int K, const mcpi_ham *hams, const cplx *const *
     for (HvGroup &g : groups) {
         const mcpi_ham h0 = hams[g.members[0]];
         bool ok = g.members.size() > 1 && has_fast_kernel(g.basis->d.N, g.basis->d.M);
-        bool real = true, dense = true;
+        bool real = true, dense = true, same_zeros = true;
         for (int x : g.members) {
             const mcpi_ham h = hams[x];
             ok = ok && h->skip_zero == h0->skip_zero && h->schedule == h0->schedule && h->schedule != 1 &&
                  h->coef_bytes == h0->coef_bytes;
             real = real && h->c.real;
             dense = dense && h->c.dense;
+            same_zeros = same_zeros && h->c.zsig == h0->c.zsig && !h->c.dense;
         }
         g.batched = ok;
         if (ok) {
             g.skip = h0->skip_zero && !dense;
+            g.multi_ok = g.skip && same_zeros && g.basis->d.Map != nullptr;
             g.cbytes = h0->coef_bytes;
             g.c0.real = real ? 1 : 0;
             for (size_t k = 0; k < g.members.size(); k++) batched_member[g.members[k]] = (int)k;
@@ -801,8 +839,7 @@ static mcpi_status lanczos_core(int K, const mcpi_ham *hams, const cplx *const *
     for (HvGroup &g : groups) {
         if (!g.batched) continue;
         const mcpi_ham h0 = hams[g.members[0]];
-        if (h0->schedule == 2 || h0->schedule == 3) g.sched = h0->schedule;
-        else MCPI_CHECK(tune_group(g, hams, hin, s));
+        MCPI_CHECK(tune_group(g, hams, hin, (h0->schedule == 2 || h0->schedule == 3) ? h0->schedule : 0, s));
     }
     for (const HvGroup &g : groups) MCPI_CHECK(launch_group(g, hams, hin, s));
     hipLaunchKernelGGL(lz_dot, gv, blk, 0, s, (const LzInst *)din);

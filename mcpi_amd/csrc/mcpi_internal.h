@@ -142,6 +142,8 @@ struct HamCoefDev {
     int real;        // 1 if every coefficient above has a zero imaginary part
     int dense;       // 1 if at least half of the double-jump terms are non-zero:
                      // walking the lists would cost more than it skips
+    uint64_t zsig;   // hash of the zero pattern (m1j, ot_off, tt_off, nz_e):
+                     // equal for Hamiltonians whose non-zero terms coincide
 };
 
 // the same tables d bytes further on (instance x of a batch, or a copy of
@@ -193,6 +195,8 @@ int ham_schedule(mcpi_ham h);
 bool has_fast_kernel(int N, int M);
 mcpi_status launch_apply_batch(const BasisDev &b, int maps, const HamCoefDev &c0, int skip, int sched, int nbatch,
                                size_t cstride, const cplx *C0, cplx *out0, int64_t vstride, hipStream_t st);
+mcpi_status launch_apply_multi(const BasisDev &b, const HamCoefDev &c0, int sched, int nbatch, size_t cstride,
+                               const cplx *C0, cplx *out0, int64_t vstride, hipStream_t st);
 // schedules: 1 generic thread-per-row kernel, 2 specialised kernel reading
 // every stored map, 3 specialised kernel ranking double jumps on the fly
 mcpi_status launch_apply_raw(const BasisDev &b, int maps, const HamCoefDev &c, int skip_zero,

@@ -24,9 +24,18 @@ def _ll_instances(spec, seed):
     return [(N, M, float(10 ** rng.uniform(-1, 2))) for (N, M) in spec]
 
 
-def test_batched_matches_oracle_ll_mixed_bases(oracle_lib):
+@pytest.fixture(params=[0, 1, 2], ids=["timed", "batched_kernel", "multi_vector"])
+def batched_kernel(request):
+    mcpi_amd.set_batched_kernel(request.param)
+    yield request.param
+    mcpi_amd.set_batched_kernel(0)
+
+
+def test_batched_matches_oracle_ll_mixed_bases(oracle_lib, batched_kernel):
     """Lieb-Liniger instances over (6,9) and (8,11) with 120 steps each
-    (the small bases of the 64-instance workload), one call."""
+    (the small bases of the 64-instance workload), one call; every H.v
+    kernel choice for the batched groups (the multi-vector kernel applies:
+    Lieb-Liniger couplings share the zero pattern)."""
     spec = [(6, 9), (8, 11), (6, 9), (8, 11), (6, 9), (6, 9), (8, 11), (8, 11)]
     inst = _ll_instances(spec, 3)
     lm = 120
@@ -50,7 +59,7 @@ def test_batched_matches_oracle_ll_mixed_bases(oracle_lib):
         assert abs(abs(np.vdot(psi_o, outs[x].numpy())) - 1) < 1e-8
 
 
-def test_batched_equals_single_instance_generic():
+def test_batched_equals_single_instance_generic(batched_kernel):
     """Generic complex Ho/Hint (no zero terms to skip), three bases, mixed
     sizes: every instance's E0, steps and Ritz vector equal the one-instance
     solver's."""
@@ -75,7 +84,7 @@ def test_batched_equals_single_instance_generic():
         assert abs(abs(np.vdot(psi1.numpy(), outs[x].numpy())) - 1) < 1e-10
 
 
-def test_batched_breakdown_and_unbatched_groups(oracle_lib):
+def test_batched_breakdown_and_unbatched_groups(oracle_lib, batched_kernel):
     """Bases smaller than lm (N=1, M=6: nc = 6; M=1: nc = 1) run next to
     full-length instances; a group whose members disagree on zero-skipping
     runs its instances one by one; an M outside the specialised kernels
@@ -120,3 +129,31 @@ def test_batched_empty_and_argument_errors():
         mcpi_amd.lanczos_batched([H], [mcpi_amd.DeviceArray(b.nc + 1)], 10)
     with pytest.raises(mcpi_amd.MCPIError):
         mcpi_amd.lanczos_batched([H], [mcpi_amd.DeviceArray.from_numpy(_start(b.nc, 1))], 0)
+
+
+def test_multi_vector_sparse_generic(oracle_lib, batched_kernel):
+    """Generic complex Hamiltonians with a common pattern of exact zeros in
+    Hint and Ho (so the multi-vector kernel applies to non-real
+    coefficients), five instances (one partial group of four), full and
+    single maps."""
+    N, M = 5, 7
+    rng = np.random.default_rng(8)
+    mask = rng.random((M,) * 4) < 0.6
+    mask = mask & mask.transpose(1, 0, 2, 3) & mask.transpose(0, 1, 3, 2) & mask.transpose(2, 3, 0, 1)
+    mho = rng.random((M, M)) < 0.5
+    mho = mho & mho.T
+    o = oracle.Basis(N, M)
+    for maps in ("full", "single"):
+        b = mcpi_amd.Basis(N, M, maps)
+        hams, starts, ref = [], [], []
+        for x in range(5):
+            C0, Ho, Hint = oracle.random_inputs(N, M, seed=60 + x)
+            Hint = np.where(mask, Hint, 0)
+            Ho = np.where(mho, Ho, 0)
+            hams.append(mcpi_amd.Hamiltonian(b, Ho, Hint))
+            starts.append(mcpi_amd.DeviceArray.from_numpy(C0))
+            ref.append(o.ground(40, C0, Ho, Hint)[0])
+        note(f"multi-vector sparse generic maps={maps}")
+        E, steps = mcpi_amd.lanczos_batched(hams, starts, 40)
+        for x in range(5):
+            assert abs(E[x] - ref[x]) < 1e-10, (maps, x, E[x], ref[x])
