@@ -208,7 +208,12 @@ mcpi_status mcpi_hip_ham_set_skip_zero(mcpi_ham h, int on);
 /* H.v kernel schedule: 0 = auto (the first H.v times the applicable
  * schedules and keeps the fastest), 1 = generic thread-per-row kernel,
  * 2 = kernel specialised on M reading every stored map, 3 = specialised
- * kernel ranking double jumps on the fly even when the maps are stored */
+ * kernel ranking double jumps on the fly even when the maps are stored,
+ * 4 = specialised kernel with zero-skipping reading the pattern tables: the
+ * targets of the non-zero double-jump terms only, built once per basis and
+ * zero pattern on first use and kept with the basis (sparse Hamiltonians
+ * with Map stored; falls back to 2 otherwise, or when device memory is
+ * short, or on a stream being captured before the tables exist) */
 mcpi_status mcpi_hip_ham_set_schedule(mcpi_ham h, int schedule);
 /* schedule in use (after autotuning; 0 before the first H.v) */
 mcpi_status mcpi_hip_ham_get_schedule(mcpi_ham h, int *schedule);

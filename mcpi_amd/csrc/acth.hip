@@ -416,15 +416,49 @@ mcpi_status launch_apply_raw(const BasisDev &b, int maps, const HamCoefDev &c, i
 // outside graph capture) times every applicable schedule with HIP events on
 // the caller's stream and keeps the fastest.  All schedules compute the same
 // result; the last one timed leaves its output in `out`.
+// schedule 4 applies: zero-skipping on a sparse Hamiltonian with Map stored
+static bool pattern_schedule_ok(mcpi_ham h)
+{
+    const BasisDev &b = h->basis->d;
+    return h->skip_zero && !h->c.dense && h->basis->maps >= MCPI_MAPS_SINGLE && has_fast_kernel(b.N, b.M);
+}
+
+// one H.v with schedule `sched`; 4 = specialised kernel reading the pattern
+// tables of the Hamiltonian's zero pattern (built on first use unless the
+// stream is capturing or memory is short; schedule 2 otherwise)
+static mcpi_status launch_sched(mcpi_ham h, int sched, const cplx *C, cplx *out, hipStream_t st)
+{
+    const BasisDev &b = h->basis->d;
+    if (sched == 4) {
+        const PatternTables *pt = nullptr;
+        if (pattern_schedule_ok(h)) {
+            hipStreamCaptureStatus cs = hipStreamCaptureStatusNone;
+            MCPI_HIP(hipStreamIsCapturing(st, &cs));
+            pt = basis_pattern(h->basis, h->c, cs == hipStreamCaptureStatusNone, st);
+        }
+        if (pt) {
+            BasisDev bd = b;
+            bd.patOT = pt->tabOT;
+            bd.patTT = pt->tabTT;
+            bd.ppoffOT = pt->offOT;
+            bd.ppoffTT = pt->offTT;
+            return launch_apply_raw(bd, h->basis->maps, h->c, 1, 2, C, out, st);
+        }
+        sched = 2;
+    }
+    return launch_apply_raw(b, h->basis->maps, h->c, h->skip_zero, sched, C, out, st);
+}
+
 static mcpi_status autotune(mcpi_ham h, const cplx *C, cplx *out, hipStream_t st)
 {
     const BasisDev &b = h->basis->d;
     const int maps = h->basis->maps;
-    int cands[3], n = 0;
+    int cands[4], n = 0;
     cands[n++] = 1;
     if (has_fast_kernel(b.N, b.M)) {
         cands[n++] = 2;
         if (maps == MCPI_MAPS_FULL) cands[n++] = 3;
+        if (pattern_schedule_ok(h) && basis_pattern(h->basis, h->c, true, st)) cands[n++] = 4;
     }
     int best = cands[n - 1];
     if (n > 1) {
@@ -433,9 +467,9 @@ static mcpi_status autotune(mcpi_ham h, const cplx *C, cplx *out, hipStream_t st
         MCPI_HIP(hipEventCreate(&e1));
         float best_ms = 1e30f;
         for (int x = 0; x < n; x++) {
-            MCPI_CHECK(launch_apply_raw(b, maps, h->c, h->skip_zero, cands[x], C, out, st));  // warm-up
+            MCPI_CHECK(launch_sched(h, cands[x], C, out, st));  // warm-up
             MCPI_HIP(hipEventRecord(e0, st));
-            MCPI_CHECK(launch_apply_raw(b, maps, h->c, h->skip_zero, cands[x], C, out, st));
+            MCPI_CHECK(launch_sched(h, cands[x], C, out, st));
             MCPI_HIP(hipEventRecord(e1, st));
             MCPI_HIP(hipEventSynchronize(e1));
             float ms = 0;
@@ -461,7 +495,7 @@ mcpi_status launch_apply(mcpi_ham h, const cplx *C, cplx *out, hipStream_t st)
         }
         sched = t ? t : (has_fast_kernel(h->basis->d.N, h->basis->d.M) ? 2 : 1);
     }
-    return launch_apply_raw(h->basis->d, h->basis->maps, h->c, h->skip_zero, sched, C, out, st);
+    return launch_sched(h, sched, C, out, st);
 }
 
 int ham_schedule(mcpi_ham h) { return h->schedule ? h->schedule : h->tuned[h->skip_zero ? 1 : 0]; }

@@ -259,6 +259,33 @@ __device__ inline cplx chunk_list(const int32_t *__restrict__ slot, int nact4, c
     return part;
 }
 
+// the listed entries of one jump read from the pattern tables: four
+// targets per 16-byte load (slot group g = entries 4g .. 4g+3 of the list)
+template <int M, bool RC>
+__device__ inline cplx chunk_pat(const int32_t *__restrict__ slot, int nact4, const cplx *__restrict__ C,
+                                 const RowState<M> &r, const double *sq, const int32_t *ents, const cplx *coefs,
+                                 int n)
+{
+    cplx part = cmake(0, 0);
+    for (int t = 0; t < n; t += 4) {
+        const Piece4 g = *reinterpret_cast<const Piece4 *>(slot + (int64_t)(t >> 2) * nact4);
+        double x[4];
+        cplx cf[4];
+#pragma unroll
+        for (int u = 0; u < 4; u++) {
+            const bool ok = t + u < n;  // uniform
+            const int e = ok ? ents[t + u] : 0;
+            const int q = e % M, l = e / M;
+            const int vq = occ<M>(r, q), vl = occ<M>(r, l);
+            x[u] = ok ? ((q == l) ? sq[vq + 1] * sq[vq + 2] : sq[vq + 1] * sq[vl + 1]) : 0.0;
+            cf[u] = ok ? coefs[t + u] : cmake(0, 0);
+        }
+#pragma unroll
+        for (int u = 0; u < 4; u++) part = term<RC>(cf[u], x[u], ld_c(C + (int64_t)g.x[u]), part);
+    }
+    return part;
+}
+
 // ---------------------------------------------------------------- kernel --
 
 // Occupancy: the zero-skipping kernels fit in 128 VGPRs (4 waves per SIMD);
@@ -274,7 +301,10 @@ __device__ inline cplx chunk_list(const int32_t *__restrict__ slot, int nact4, c
 // the XCD remap: the nbatch blocks of one row range run back to back on the
 // same XCD, so the row range's tables are read from HBM once and served to
 // the other instances from that XCD's L2.
-template <int M, bool MAP1, bool DMAP, bool SKIP, bool RC, bool BATCH>
+// PAT (with SKIP): double-jump targets of the listed terms from the
+// pattern tables of the Hamiltonian's zero pattern (no ranking, no packed
+// table lookups).
+template <int M, bool MAP1, bool DMAP, bool SKIP, bool RC, bool BATCH, bool PAT>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(SKIP ? 4 : 2))) void k_acth_fast(
     BasisDev b, HamCoefDev c0, const cplx *__restrict__ C0, cplx *__restrict__ out0, int nbatch, size_t cstride,
     int64_t vstride)
@@ -292,7 +322,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(SKIP ? 4 : 
     uint32_t *Dl = Bt + M * (b.N + 1);                  // 2*M*blockDim (SKIP with ranking only)
     const int lane = threadIdx.x, T = blockDim.x;
     for (int x = threadIdx.x; x < b.N + 3; x += blockDim.x) sq[x] = sqrt((double)x);
-    if (!MAP1 || !DMAP)
+    if (!MAP1 || (!DMAP && !PAT))
         for (int x = threadIdx.x; x < M * (b.N + 1); x += blockDim.x) Bt[x] = (uint32_t)b.B[x];
     __syncthreads();
     const int64_t i = (int64_t)rb * blockDim.x + threadIdx.x;
@@ -371,8 +401,47 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(SKIP ? 4 : 
     const int lw = threadIdx.x & 63;
     const uint64_t below = lw ? (~0ull >> (64 - lw)) : 0ull;
     const int64_t wv = i >> 6;
+    if (PAT) {
+        // pattern tables: slots only for jumps with a non-empty list
+        int64_t pb = b.ppoffOT[wv];
+        for (int k = 0; k < M; k++) {
+            const int beg = c.ot_off[k], n_nz = c.ot_off[k + 1] - beg;
+            if (n_nz == 0) continue;
+            const int vk = occ<M>(r, k);
+            const bool act = vk >= 2;
+            const uint64_t m = __ballot(act);
+            const int nact = __popcll(m);
+            const int32_t *slot = b.patOT + pb + 4 * __popcll(m & below);
+            pb += (int64_t)nact * ((n_nz + 3) & ~3);
+            if (!act) continue;
+            const cplx part = chunk_pat<M, RC>(slot, 4 * nact, C, r, sq, c.nz_e + beg, c.nz_c + beg, n_nz);
+            const double f = sq[vk] * sq[vk - 1];
+            acc.re = fma(f, part.re, acc.re);
+            acc.im = fma(f, part.im, acc.im);
+        }
+        pb = b.ppoffTT[wv];
+        int p = 0;
+        for (int k = 0; k < M; k++) {
+            const int vk = occ<M>(r, k);
+            for (int s = k + 1; s < M; s++, p++) {
+                const int beg = c.tt_off[p], n_nz = c.tt_off[p + 1] - beg;
+                if (n_nz == 0) continue;
+                const int vs = occ<M>(r, s);
+                const bool act = vk >= 1 && vs >= 1;
+                const uint64_t m = __ballot(act);
+                const int nact = __popcll(m);
+                const int32_t *slot = b.patTT + pb + 4 * __popcll(m & below);
+                pb += (int64_t)nact * ((n_nz + 3) & ~3);
+                if (!act) continue;
+                const cplx part = chunk_pat<M, RC>(slot, 4 * nact, C, r, sq, c.nz_e + beg, c.nz_c + beg, n_nz);
+                const double f = sq[vk] * sq[vs];
+                acc.re = fma(f, part.re, acc.re);
+                acc.im = fma(f, part.im, acc.im);
+            }
+        }
+    }
     // same-orbital double jumps: rules 3, 6
-    {
+    if (!PAT) {
         int64_t pb = DMAP ? b.poffOT[wv] : 0;
         for (int k = 0; k < M; k++) {
             const int n_nz = SKIP ? c.ot_off[k + 1] - c.ot_off[k] : 1;
@@ -402,7 +471,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(SKIP ? 4 : 
         }
     }
     // two-orbital double jumps: rules 7, 9
-    {
+    if (!PAT) {
         int64_t pb = DMAP ? b.poffTT[wv] : 0;
         int p = 0;
         for (int k = 0; k < M; k++) {
@@ -451,25 +520,29 @@ inline mcpi_status launch_fast_m(const BasisDev &b, int maps, const HamCoefDev &
 {
     dim3 grid((unsigned)((b.nc + 255) / 256) * (unsigned)(nbatch > 0 ? nbatch : 1)), blk(256);
     size_t sm = sizeof(double) * (b.N + 3) + sizeof(uint32_t) * M * (b.N + 1) + 16;
-    if (skip) sm += sizeof(uint32_t) * 2 * M * 256;
     const bool map1 = maps >= MCPI_MAPS_SINGLE;
+    if (skip && !(map1 && b.patTT && b.patOT)) sm += sizeof(uint32_t) * 2 * M * 256;
     const bool dmap = maps == MCPI_MAPS_FULL && !rank_doubles && b.packTT && b.packOT;
     const bool real = c.real;
-#define MCPI_L(A, B, S, R)                                                                                  \
+    const bool pat = skip && map1 && b.patTT && b.patOT;
+#define MCPI_LP(A, B, S, R, P)                                                                              \
     {                                                                                                       \
         if (nbatch > 0)                                                                                     \
-            hipLaunchKernelGGL((k_acth_fast<M, A, B, S, R, true>), grid, blk, sm, st, b, c, C, out, nbatch, \
-                               cstride, vstride);                                                                       \
+            hipLaunchKernelGGL((k_acth_fast<M, A, B, S, R, true, P>), grid, blk, sm, st, b, c, C, out,      \
+                               nbatch, cstride, vstride);                                                   \
         else                                                                                                \
-            hipLaunchKernelGGL((k_acth_fast<M, A, B, S, R, false>), grid, blk, sm, st, b, c, C, out, nbatch, \
-                               cstride, vstride);                                                                       \
+            hipLaunchKernelGGL((k_acth_fast<M, A, B, S, R, false, P>), grid, blk, sm, st, b, c, C, out,     \
+                               nbatch, cstride, vstride);                                                   \
     }
+#define MCPI_L(A, B, S, R) MCPI_LP(A, B, S, R, false)
 #define MCPI_L2(A, B, S) { if (real) MCPI_L(A, B, S, true) else MCPI_L(A, B, S, false) }
-    if (map1 && dmap) { if (skip) MCPI_L2(true, true, true) else MCPI_L2(true, true, false) }
+    if (pat) { if (real) MCPI_LP(true, false, true, true, true) else MCPI_LP(true, false, true, false, true) }
+    else if (map1 && dmap) { if (skip) MCPI_L2(true, true, true) else MCPI_L2(true, true, false) }
     else if (map1) { if (skip) MCPI_L2(true, false, true) else MCPI_L2(true, false, false) }
     else { if (skip) MCPI_L2(false, false, true) else MCPI_L2(false, false, false) }
 #undef MCPI_L2
 #undef MCPI_L
+#undef MCPI_LP
     MCPI_HIP(hipGetLastError());
     return MCPI_OK;
 }

@@ -65,7 +65,41 @@ __device__ inline void chunk_list_multi(const int32_t *__restrict__ slot, int na
     }
 }
 
-template <int M, int NV, bool DMAP, bool RC>
+// the listed entries of one jump from the pattern tables (four targets per
+// 16-byte load), for NV instances
+template <int M, int NV, bool RC>
+__device__ inline void chunk_pat_multi(const int32_t *__restrict__ slot, int nact4, const RowState<M> &r,
+                                       const double *sq, const int32_t *ents, const cplx *coefs0, size_t cstride,
+                                       const cplx *__restrict__ C0, int64_t vstride, int nv, int n,
+                                       cplx (&part)[NV])
+{
+    for (int t = 0; t < n; t += 4) {
+        const Piece4 g = *reinterpret_cast<const Piece4 *>(slot + (int64_t)(t >> 2) * nact4);
+        double x[4];
+#pragma unroll
+        for (int u = 0; u < 4; u++) {
+            const bool ok = t + u < n;  // uniform
+            const int e = ok ? ents[t + u] : 0;
+            const int q = e % M, l = e / M;
+            const int vq = occ<M>(r, q), vl = occ<M>(r, l);
+            x[u] = ok ? ((q == l) ? sq[vq + 1] * sq[vq + 2] : sq[vq + 1] * sq[vl + 1]) : 0.0;
+        }
+#pragma unroll
+        for (int v = 0; v < NV; v++) {
+            if (v >= nv) break;   // uniform
+            const cplx *cf = shifted(coefs0, (size_t)v * cstride);
+            const cplx *Cv = C0 + (int64_t)v * vstride;
+#pragma unroll
+            for (int u = 0; u < 4; u++) {
+                const cplx c = (t + u < n) ? cf[t + u] : cmake(0, 0);
+                part[v] = term<RC>(c, x[u], ld_c(Cv + (int64_t)g.x[u]), part[v]);
+            }
+        }
+    }
+}
+
+// PAT: double-jump targets of the listed terms from the pattern tables
+template <int M, int NV, bool DMAP, bool RC, bool PAT>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void k_acth_multi(
     BasisDev b, HamCoefDev c0, const cplx *__restrict__ C0, cplx *__restrict__ out0, int nbatch, size_t cstride,
     int64_t vstride)
@@ -85,7 +119,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void k
     uint32_t *Dl = Bt + M * (b.N + 1);                  // 2*M*blockDim (ranked doubles)
     const int lane = threadIdx.x, T = blockDim.x;
     for (int x = threadIdx.x; x < b.N + 3; x += blockDim.x) sq[x] = sqrt((double)x);
-    if (!DMAP)
+    if (!DMAP && !PAT)
         for (int x = threadIdx.x; x < M * (b.N + 1); x += blockDim.x) Bt[x] = (uint32_t)b.B[x];
     __syncthreads();
     const int64_t i = (int64_t)rb * blockDim.x + threadIdx.x;
@@ -151,8 +185,61 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void k
     const uint64_t below = lw ? (~0ull >> (64 - lw)) : 0ull;
     const int64_t wv = i >> 6;
     const cplx *nzc = shifted(c0.nz_c, cbase);
+    if (PAT) {
+        // pattern tables: slots only for jumps with a non-empty list
+        int64_t pb = b.ppoffOT[wv];
+        for (int k = 0; k < M; k++) {
+            const int beg = c0.ot_off[k], n_nz = c0.ot_off[k + 1] - beg;
+            if (n_nz == 0) continue;
+            const int vk = occ<M>(r, k);
+            const bool act = vk >= 2;
+            const uint64_t m = __ballot(act);
+            const int nact = __popcll(m);
+            const int32_t *slot = b.patOT + pb + 4 * __popcll(m & below);
+            pb += (int64_t)nact * ((n_nz + 3) & ~3);
+            if (!act) continue;
+            cplx part[NV];
+#pragma unroll
+            for (int v = 0; v < NV; v++) part[v] = cmake(0, 0);
+            chunk_pat_multi<M, NV, RC>(slot, 4 * nact, r, sq, c0.nz_e + beg, nzc + beg, cstride, C, vstride, nv,
+                                       n_nz, part);
+            const double f = sq[vk] * sq[vk - 1];
+#pragma unroll
+            for (int v = 0; v < NV; v++) {
+                acc[v].re = fma(f, part[v].re, acc[v].re);
+                acc[v].im = fma(f, part[v].im, acc[v].im);
+            }
+        }
+        pb = b.ppoffTT[wv];
+        int p = 0;
+        for (int k = 0; k < M; k++) {
+            const int vk = occ<M>(r, k);
+            for (int s = k + 1; s < M; s++, p++) {
+                const int beg = c0.tt_off[p], n_nz = c0.tt_off[p + 1] - beg;
+                if (n_nz == 0) continue;
+                const int vs = occ<M>(r, s);
+                const bool act = vk >= 1 && vs >= 1;
+                const uint64_t m = __ballot(act);
+                const int nact = __popcll(m);
+                const int32_t *slot = b.patTT + pb + 4 * __popcll(m & below);
+                pb += (int64_t)nact * ((n_nz + 3) & ~3);
+                if (!act) continue;
+                cplx part[NV];
+#pragma unroll
+                for (int v = 0; v < NV; v++) part[v] = cmake(0, 0);
+                chunk_pat_multi<M, NV, RC>(slot, 4 * nact, r, sq, c0.nz_e + beg, nzc + beg, cstride, C, vstride,
+                                           nv, n_nz, part);
+                const double f = sq[vk] * sq[vs];
+#pragma unroll
+                for (int v = 0; v < NV; v++) {
+                    acc[v].re = fma(f, part[v].re, acc[v].re);
+                    acc[v].im = fma(f, part[v].im, acc[v].im);
+                }
+            }
+        }
+    }
     // same-orbital double jumps: rules 3, 6
-    {
+    if (!PAT) {
         int64_t pb = DMAP ? b.poffOT[wv] : 0;
         for (int k = 0; k < M; k++) {
             const int n_nz = c0.ot_off[k + 1] - c0.ot_off[k];
@@ -182,7 +269,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void k
         }
     }
     // two-orbital double jumps: rules 7, 9
-    {
+    if (!PAT) {
         int64_t pb = DMAP ? b.poffTT[wv] : 0;
         int p = 0;
         for (int k = 0; k < M; k++) {
@@ -238,23 +325,22 @@ inline mcpi_status launch_multi_m(const BasisDev &b, const HamCoefDev &c0, int r
     const unsigned G = (unsigned)((nbatch + NV - 1) / NV);
     dim3 grid((unsigned)((b.nc + 255) / 256) * G), blk(256);
     size_t sm = sizeof(double) * (b.N + 3) + sizeof(uint32_t) * M * (b.N + 1) + 16;
-    const bool dmap = !rank_doubles && b.packTT && b.packOT;
-    if (!dmap) sm += sizeof(uint32_t) * 2 * M * 256;
-    if (dmap) {
-        if (c0.real)
-            hipLaunchKernelGGL((k_acth_multi<M, NV, true, true>), grid, blk, sm, st, b, c0, C0, out0, nbatch, cstride,
-                               vstride);
-        else
-            hipLaunchKernelGGL((k_acth_multi<M, NV, true, false>), grid, blk, sm, st, b, c0, C0, out0, nbatch, cstride,
-                               vstride);
+    const bool pat = b.patTT && b.patOT;
+    const bool dmap = !pat && !rank_doubles && b.packTT && b.packOT;
+    if (!dmap && !pat) sm += sizeof(uint32_t) * 2 * M * 256;
+#define MCPI_LM(D, R, P)                                                                                       \
+    hipLaunchKernelGGL((k_acth_multi<M, NV, D, R, P>), grid, blk, sm, st, b, c0, C0, out0, nbatch, cstride, vstride)
+    if (pat) {
+        if (c0.real) MCPI_LM(false, true, true);
+        else MCPI_LM(false, false, true);
+    } else if (dmap) {
+        if (c0.real) MCPI_LM(true, true, false);
+        else MCPI_LM(true, false, false);
     } else {
-        if (c0.real)
-            hipLaunchKernelGGL((k_acth_multi<M, NV, false, true>), grid, blk, sm, st, b, c0, C0, out0, nbatch,
-                               cstride, vstride);
-        else
-            hipLaunchKernelGGL((k_acth_multi<M, NV, false, false>), grid, blk, sm, st, b, c0, C0, out0, nbatch,
-                               cstride, vstride);
+        if (c0.real) MCPI_LM(false, true, false);
+        else MCPI_LM(false, false, false);
     }
+#undef MCPI_LM
     MCPI_HIP(hipGetLastError());
     return MCPI_OK;
 }

@@ -370,6 +370,99 @@ __global__ void k_packed_fill(mcpi::BasisDev b)
     }
 }
 
+// ---- pattern tables (targets of the non-zero terms of one zero pattern) --
+//
+// For a Hamiltonian whose folded double-jump coefficients are mostly zero
+// (Lieb-Liniger keeps about 3 of the 78 entries of a chunk), the zero-
+// skipping H.v only needs the targets of the listed entries.  These tables
+// hold exactly those, in the wave-packed slot layout: for wavefront w and
+// each jump with a non-empty list that is active in at least one lane, one
+// slot [g][a][4] = target of list entry t = 4g + u of the a-th active lane
+// (t < n; padding entries hold the row's own index), n rounded up to 4.
+
+__device__ inline int round4(int n) { return (n + 3) & ~3; }
+
+__global__ void k_pattern_counts(mcpi::BasisDev b, HamCoefDev c, int32_t *__restrict__ cnt_ot,
+                                 int32_t *__restrict__ cnt_tt)
+{
+    const int M = b.M;
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const int64_t w = i >> 6, nw = (b.nc + 63) >> 6;
+    const bool valid = i < b.nc;
+    uint32_t m1 = 0, m2 = 0;
+    if (valid)
+        for (int j = 0; j < M; j++) {
+            const int x = b.IF[i * M + j];
+            m1 |= (uint32_t)(x >= 1) << j;
+            m2 |= (uint32_t)(x >= 2) << j;
+        }
+    int64_t ot = 0, tt = 0;
+    for (int k = 0; k < M; k++) {
+        const int n = c.ot_off[k + 1] - c.ot_off[k];
+        if (n) ot += (int64_t)__popcll(__ballot(valid && ((m2 >> k) & 1u))) * round4(n);
+    }
+    int p = 0;
+    for (int k = 0; k < M; k++)
+        for (int s = k + 1; s < M; s++, p++) {
+            const int n = c.tt_off[p + 1] - c.tt_off[p];
+            if (n) tt += (int64_t)__popcll(__ballot(valid && ((m1 >> k) & 1u) && ((m1 >> s) & 1u))) * round4(n);
+        }
+    if ((threadIdx.x & 63) == 0 && w < nw) {
+        cnt_ot[w] = (int32_t)ot;
+        cnt_tt[w] = (int32_t)tt;
+    }
+}
+
+template <bool TT>
+__global__ void k_pattern_fill(mcpi::BasisDev b, HamCoefDev c, const int64_t *__restrict__ off,
+                               int32_t *__restrict__ tab)
+{
+    extern __shared__ int smem[];
+    const int M = b.M, N = b.N;
+    int32_t *Bt = smem;                                                 // M*(N+1)
+    int *P = smem + M * (N + 1) + threadIdx.x * (MCPI_MAXM + 1);        // prefix counts
+    for (int t = threadIdx.x; t < M * (N + 1); t += blockDim.x) Bt[t] = b.B[t];
+    __syncthreads();
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const int64_t w = i >> 6, nw = (b.nc + 63) >> 6;
+    if (w >= nw) return;   // whole wavefronts past the last row
+    const bool valid = i < b.nc;
+    const int lane = threadIdx.x & 63;
+    const uint64_t below = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
+    uint32_t m1 = 0, m2 = 0;
+    int acc = 0;
+    for (int j = 0; j < M; j++) {
+        const int x = valid ? b.IF[i * M + j] : 0;
+        P[j] = acc;
+        acc += x;
+        m1 |= (uint32_t)(x >= 1) << j;
+        m2 |= (uint32_t)(x >= 2) << j;
+    }
+    int64_t base = off[w];
+    int p = 0;
+    for (int k = 0; k < M; k++) {
+        for (int s = TT ? k + 1 : k; s < (TT ? M : k + 1); s++, p++) {
+            const int32_t *lo = TT ? c.tt_off + p : c.ot_off + k;
+            const int beg = lo[0], n = lo[1] - lo[0];
+            if (n == 0) continue;
+            const bool act = valid && (TT ? (((m1 >> k) & 1u) && ((m1 >> s) & 1u)) : ((m2 >> k) & 1u));
+            const uint64_t m = __ballot(act);
+            if (m == 0) continue;
+            const int nact = __popcll(m), my = __popcll(m & below), n4 = round4(n);
+            if (act)
+                for (int t = 0; t < n4; t++) {
+                    int32_t val = (int32_t)i;
+                    if (t < n) {
+                        const int e = c.nz_e[beg + t];
+                        val = (int32_t)rank_moved(P, Bt, N, M, b.nc, k, s, e / M, e % M);
+                    }
+                    tab[base + ((int64_t)(t >> 2) * nact + my) * 4 + (t & 3)] = val;
+                }
+            base += (int64_t)nact * n4;
+        }
+    }
+}
+
 }  // namespace mcpi
 
 using namespace mcpi;
@@ -418,6 +511,11 @@ static void free_basis(mcpi_basis b)
                     b->d.B,      b->d.NCmat,  b->d.packOT, b->d.packTT, b->d.poffOT,   b->d.poffTT};
     for (void *p : ptrs)
         if (p) (void)hipFree(p);
+    for (PatternTables &t : b->patterns) {
+        void *q[] = {t.tabOT, t.tabTT, t.offOT, t.offTT};
+        for (void *p : q)
+            if (p) (void)hipFree(p);
+    }
     delete b;
 }
 
@@ -566,6 +664,75 @@ static mcpi_status basis_build_maps(mcpi_basis b, int maps)
 }
 
 namespace mcpi {
+
+// Pattern tables are kept for at most this many zero patterns per basis
+// (least recently used dropped first) and only while they leave this much
+// device memory free.
+#define MCPI_PATTERNS_PER_BASIS 4
+#define MCPI_PATTERN_RESERVE ((int64_t)4 << 30)
+
+static void free_pattern(PatternTables &t)
+{
+    void *q[] = {t.tabOT, t.tabTT, t.offOT, t.offTT};
+    for (void *p : q)
+        if (p) (void)hipFree(p);
+    t = PatternTables();
+}
+
+const PatternTables *basis_pattern(mcpi_basis b, const HamCoefDev &c, bool may_build, hipStream_t st)
+{
+    for (PatternTables &t : b->patterns)
+        if (t.zsig == c.zsig) {
+            t.last_use = ++b->use_clock;
+            return &t;
+        }
+    if (!may_build || b->d.M > MCPI_MAXM) return nullptr;
+    const int M = b->d.M, N = b->d.N;
+    const int64_t nc = b->d.nc, nw = (nc + 63) / 64;
+    PatternTables t;
+    t.zsig = c.zsig;
+    int32_t *cnt = nullptr;
+    if (hipMalloc(&cnt, sizeof(int32_t) * 2 * nw) != hipSuccess) {
+        (void)hipGetLastError();
+        return nullptr;
+    }
+    const int64_t grid = (nw * 64 + 255) / 256;
+    bool ok = true;
+    hipLaunchKernelGGL(k_pattern_counts, dim3(grid), dim3(256), 0, st, b->d, c, cnt, cnt + nw);
+    ok = hipGetLastError() == hipSuccess && hipStreamSynchronize(st) == hipSuccess;
+    ok = ok && hipMalloc(&t.offOT, sizeof(int64_t) * (nw + 1)) == hipSuccess &&
+         hipMalloc(&t.offTT, sizeof(int64_t) * (nw + 1)) == hipSuccess;
+    ok = ok && scan_to_strides(nw, cnt, 1, t.offOT, &t.lenOT) == MCPI_OK &&
+         scan_to_strides(nw, cnt + nw, 1, t.offTT, &t.lenTT) == MCPI_OK;
+    (void)hipFree(cnt);
+    size_t free_b = 0, total_b = 0;
+    ok = ok && hipMemGetInfo(&free_b, &total_b) == hipSuccess;
+    t.bytes = 4 * (t.lenOT + t.lenTT) + 16 * (nw + 1);
+    ok = ok && (int64_t)free_b > 4 * (t.lenOT + t.lenTT) + MCPI_PATTERN_RESERVE;
+    ok = ok && hipMalloc(&t.tabOT, 4 * std::max<int64_t>(t.lenOT, 1)) == hipSuccess &&
+         hipMalloc(&t.tabTT, 4 * std::max<int64_t>(t.lenTT, 1)) == hipSuccess;
+    if (ok) {
+        const size_t sm = sizeof(int) * (M * (N + 1) + 256 * (MCPI_MAXM + 1));
+        if (t.lenOT) hipLaunchKernelGGL(k_pattern_fill<false>, dim3(grid), dim3(256), sm, st, b->d, c, t.offOT, t.tabOT);
+        if (t.lenTT) hipLaunchKernelGGL(k_pattern_fill<true>, dim3(grid), dim3(256), sm, st, b->d, c, t.offTT, t.tabTT);
+        ok = hipGetLastError() == hipSuccess && hipStreamSynchronize(st) == hipSuccess;
+    }
+    if (!ok) {
+        (void)hipGetLastError();
+        free_pattern(t);
+        return nullptr;
+    }
+    if ((int)b->patterns.size() >= MCPI_PATTERNS_PER_BASIS) {
+        size_t old = 0;
+        for (size_t x = 1; x < b->patterns.size(); x++)
+            if (b->patterns[x].last_use < b->patterns[old].last_use) old = x;
+        free_pattern(b->patterns[old]);
+        b->patterns.erase(b->patterns.begin() + old);
+    }
+    t.last_use = ++b->use_clock;
+    b->patterns.push_back(t);
+    return &b->patterns.back();
+}
 
 // Upload of caller-provided tables (reference-shaped entry points).  int32
 // strides of the reference become int64 with a closing entry.

@@ -624,12 +624,29 @@ struct HvGroup {
 static mcpi_status launch_group(const HvGroup &g, const mcpi_ham *hams, const std::vector<LzInst> &hin,
                                 hipStream_t s)
 {
-    if (g.batched && g.mode == 2)
-        return launch_apply_multi(g.basis->d, g.c0, g.sched, (int)g.members.size(), g.cbytes, g.vin, g.HC,
-                                  g.basis->d.nc, s);
-    if (g.batched)
-        return launch_apply_batch(g.basis->d, g.basis->maps, g.c0, g.skip, g.sched, (int)g.members.size(), g.cbytes,
-                                  g.vin, g.HC, g.basis->d.nc, s);
+    if (g.batched) {
+        BasisDev bd = g.basis->d;
+        int sched = g.sched;
+        if (sched == 4) {
+            // pattern tables of the group's common zero pattern (built by the
+            // tuner before any capture)
+            hipStreamCaptureStatus cs = hipStreamCaptureStatusNone;
+            MCPI_HIP(hipStreamIsCapturing(s, &cs));
+            const PatternTables *pt =
+                basis_pattern(g.basis, hams[g.members[0]]->c, cs == hipStreamCaptureStatusNone, s);
+            if (pt) {
+                bd.patOT = pt->tabOT;
+                bd.patTT = pt->tabTT;
+                bd.ppoffOT = pt->offOT;
+                bd.ppoffTT = pt->offTT;
+            }
+            sched = 2;
+        }
+        if (g.mode == 2)
+            return launch_apply_multi(bd, g.c0, sched, (int)g.members.size(), g.cbytes, g.vin, g.HC, bd.nc, s);
+        return launch_apply_batch(bd, g.basis->maps, g.c0, g.skip, sched, (int)g.members.size(), g.cbytes, g.vin,
+                                  g.HC, bd.nc, s);
+    }
     for (int x : g.members) MCPI_CHECK(launch_apply(hams[x], hin[x].vin, hin[x].HC, s));
     return MCPI_OK;
 }
@@ -659,10 +676,12 @@ static mcpi_status tune_group(HvGroup &g, const mcpi_ham *hams, const std::vecto
         if (mode == 2 && !g.multi_ok) continue;
         if (g_batched_kernel == 1 && mode == 2) continue;
         if (g_batched_kernel == 2 && mode == 1 && g.multi_ok) continue;
-        for (int sc = 2; sc <= 3; sc++) {
+        for (int sc = 2; sc <= 4; sc++) {
             if (fixed && sc != fixed) continue;
             if (sc == 3 && !full && mode == 1) continue;   // without stored doubles 2 already ranks
             if (sc == 2 && !full && mode == 2) continue;
+            // 4: pattern tables, for a zero pattern all members share
+            if (sc == 4 && !(g.multi_ok && basis_pattern(g.basis, hams[g.members[0]]->c, true, s))) continue;
             cand.push_back({mode, sc});
         }
     }
@@ -839,7 +858,7 @@ static mcpi_status lanczos_core(int K, const mcpi_ham *hams, const cplx *const *
     for (HvGroup &g : groups) {
         if (!g.batched) continue;
         const mcpi_ham h0 = hams[g.members[0]];
-        MCPI_CHECK(tune_group(g, hams, hin, (h0->schedule == 2 || h0->schedule == 3) ? h0->schedule : 0, s));
+        MCPI_CHECK(tune_group(g, hams, hin, (h0->schedule >= 2 && h0->schedule <= 4) ? h0->schedule : 0, s));
     }
     for (const HvGroup &g : groups) MCPI_CHECK(launch_group(g, hams, hin, s));
     hipLaunchKernelGGL(lz_dot, gv, blk, 0, s, (const LzInst *)din);

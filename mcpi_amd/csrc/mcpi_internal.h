@@ -2,6 +2,8 @@
 #pragma once
 
 #include <hip/hip_runtime.h>
+
+#include <vector>
 #include <stdint.h>
 
 #include <string>
@@ -97,6 +99,22 @@ struct BasisDev {
     // in any of its lanes, from int offset poffOT[w] / poffTT[w]
     int32_t *packOT, *packTT;
     int64_t *poffOT, *poffTT;   // nw + 1 entries, nw = ceil(nc / 64)
+    // Pattern tables of one zero pattern (set only in the copy a launch
+    // passes): the same wave-interleaved slots, holding only the targets of
+    // the listed (non-zero) terms of each jump, in list order, padded to a
+    // multiple of four
+    int32_t *patOT, *patTT;
+    int64_t *ppoffOT, *ppoffTT;
+};
+
+// pattern tables of a basis for one zero pattern (see basis.hip)
+struct PatternTables {
+    uint64_t zsig = 0;
+    int32_t *tabOT = nullptr, *tabTT = nullptr;
+    int64_t *offOT = nullptr, *offTT = nullptr;
+    int64_t lenOT = 0, lenTT = 0;
+    int64_t bytes = 0;
+    uint64_t last_use = 0;
 };
 
 // entries of a packed slot: the M(M+1)/2 targets q <= l of a chunk, in
@@ -112,6 +130,8 @@ struct mcpi_basis_s {
     int64_t ot_len, tt_len, map_len;
     int64_t pot_len, ptt_len;   // ints in packOT / packTT
     int64_t device_bytes;
+    std::vector<mcpi::PatternTables> patterns;   // built on demand, a few zero patterns
+    uint64_t use_clock = 0;
 };
 
 // Folded coefficient tables of a Hamiltonian (device memory), produced on
@@ -193,6 +213,9 @@ __host__ __device__ inline int pair_index(int k, int s, int M)
 mcpi_status launch_apply(mcpi_ham h, const cplx *C, cplx *out, hipStream_t st);
 int ham_schedule(mcpi_ham h);
 bool has_fast_kernel(int N, int M);
+// pattern tables of basis b for the zero pattern of c; built when absent if
+// may_build and device memory allows; nullptr when not available
+const PatternTables *basis_pattern(mcpi_basis b, const HamCoefDev &c, bool may_build, hipStream_t st);
 mcpi_status launch_apply_batch(const BasisDev &b, int maps, const HamCoefDev &c0, int skip, int sched, int nbatch,
                                size_t cstride, const cplx *C0, cplx *out0, int64_t vstride, hipStream_t st);
 mcpi_status launch_apply_multi(const BasisDev &b, const HamCoefDev &c0, int sched, int nbatch, size_t cstride,

@@ -21,12 +21,16 @@ CASES = [(1, 1), (3, 1), (0, 3), (1, 3), (2, 3), (5, 5), (4, 6), (6, 4), (3, 7),
 
 def schedules(M, maps):
     """1 generic; 2 specialised (reads every stored map); 3 specialised,
-    double jumps ranked on the fly (differs from 2 only with full maps)."""
+    double jumps ranked on the fly (differs from 2 only with full maps);
+    4 specialised with the pattern tables of the zero pattern (zero-skipping
+    on a sparse Hamiltonian with Map stored; otherwise it runs as 2)."""
     out = [1]
     if 3 <= M <= 13:
         out.append(2)
         if maps == "full":
             out.append(3)
+        if maps != "none":
+            out.append(4)
     return out
 
 
@@ -186,3 +190,6 @@ def test_n16_m12_single_map_sampled_rows(oracle_lib):
     rows = np.concatenate([[0, b.nc - 1], rng.integers(0, b.nc, 3000)])
     assert_close(out[rows], oracle.acth_hash_rows(N, M, rows, C, Ho, Hint), 1e-12, "rows")
     _hermitian_check(H, b.nc, 9)
+    for sched in (2, 4):
+        H2 = mcpi_amd.Hamiltonian(b, Ho, Hint, skip_zero=True, schedule=sched)
+        assert_close(H2(C), out, 1e-12, f"schedule {sched}")
