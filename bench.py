@@ -178,7 +178,8 @@ def large_basis(reps: int = 20, lm: int = 200, quick: bool = False):
         H.apply(dC, dO)                      # schedule tuning
         ms = H.apply_timed(dC, dO, reps)
         out[label] = {"ms_per_hv": round(ms, 4), "gb_s_model": round(B / ms / 1e6, 1),
-                      "hbm_frac_model": round(B / ms / 1e6 / HBM_PEAK_GBS, 4), "schedule": H.schedule}
+                      "hbm_frac_model": round(B / ms / 1e6 / HBM_PEAK_GBS, 4), "schedule": H.schedule,
+                      "pattern_table_bytes": b.pattern_bytes()}
         H.close()
     del dO
     if quick:
@@ -233,10 +234,15 @@ def main() -> int:
     if args.copy_calibration:
         n = 1 << 30
         src, dst = mcpi_amd.DeviceArray(n // 16), mcpi_amd.DeviceArray(n // 16)
+        times = []
         for _ in range(3):
+            mcpi_amd.synchronize()
+            t = time.perf_counter()
             mcpi_amd._check(mcpi_amd.lib().mcpi_hip_memcpy_d2d(dst.ptr, src.ptr, n), "copy")
-        mcpi_amd.synchronize()
-        print(json.dumps({"copy_calibration": {"bytes_per_copy": n, "copies": 3}}))
+            mcpi_amd.synchronize()
+            times.append(time.perf_counter() - t)
+        print(json.dumps({"copy_calibration": {"bytes_per_copy": n, "copies": 3,
+                                               "read_plus_write_GB_s_best": round(2 * n / min(times) / 1e9, 1)}}))
         return 0
 
     if args.large_only:
@@ -287,6 +293,8 @@ def main() -> int:
             for _ in range(2):
                 Hx.apply(dC, dO)
             others[label] = {"ms_per_hv": round(Hx.apply_timed(dC, dO, reps), 4), "schedule": Hx.schedule}
+            if Hx.schedule == 4:
+                others[label]["pattern_table_bytes"] = b.pattern_bytes()
             agree = max(agree, float(np.linalg.norm(dO.numpy() - gpu_out) / np.linalg.norm(gpu_out)))
 
     batched = large = None

@@ -153,6 +153,8 @@ typedef struct {
     int64_t pot_len, ptt_len; /* ints of the wave-packed OT / TT tables that
                                  the H.v kernels read (full maps only; the
                                  layout is given at mcpi_hip_basis_copy_range) */
+    int64_t pattern_bytes;    /* device memory of the pattern tables built so
+                                 far for H.v schedule 4 (all zero patterns) */
 } mcpi_basis_info;
 
 const char *mcpi_hip_last_error(void);

@@ -49,7 +49,7 @@ class _BasisInfo(ct.Structure):
     _fields_ = [("N", ct.c_int), ("M", ct.c_int), ("maps", ct.c_int),
                 ("nc", ct.c_int64), ("map_len", ct.c_int64), ("ot_len", ct.c_int64),
                 ("tt_len", ct.c_int64), ("device_bytes", ct.c_int64),
-                ("pot_len", ct.c_int64), ("ptt_len", ct.c_int64)]
+                ("pot_len", ct.c_int64), ("ptt_len", ct.c_int64), ("pattern_bytes", ct.c_int64)]
 
 
 _lib = None
@@ -239,6 +239,12 @@ class Basis:
               "IF": (3, np.int32), "strideOT": (4, np.int64), "strideTT": (5, np.int64),
               "packOT": (6, np.int32), "packTT": (7, np.int32), "poffOT": (8, np.int64),
               "poffTT": (9, np.int64)}
+
+    def pattern_bytes(self) -> int:
+        """Device memory of the pattern tables built so far (schedule 4)."""
+        info = _BasisInfo()
+        _check(lib().mcpi_hip_basis_info(self.handle, ct.byref(info)), "basis_info")
+        return int(info.pattern_bytes)
 
     def copy_range(self, table: str, offset: int, count: int) -> np.ndarray:
         """``count`` entries of a device table starting at ``offset``."""

@@ -862,6 +862,8 @@ mcpi_status mcpi_hip_basis_info(mcpi_basis b, mcpi_basis_info *info)
     info->device_bytes = b->device_bytes;
     info->pot_len = b->pot_len;
     info->ptt_len = b->ptt_len;
+    info->pattern_bytes = 0;
+    for (const PatternTables &t : b->patterns) info->pattern_bytes += t.bytes;
     return MCPI_OK;
 }
 

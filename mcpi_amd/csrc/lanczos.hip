@@ -375,8 +375,12 @@ __global__ void lz_fin_ortho(const LanczosState *sts, const LzInst *in, int lm)
     }
 }
 
-// 8: HC -= sum_t v_t o_t;  partial ||HC||^2 for the next beta.  The o_t are
+// 8: HC -= sum_t v_t o_t;  partial ||HC||^2 for the next beta.  A thread
+// owns LZ_RR rows of each block step (rows j + r * LZ_T) and walks the basis
+// vectors with one load per row in flight, two vectors at a time, so every
+// lane keeps 2 * LZ_RR independent 16-byte loads outstanding.  The o_t are
 // the same for every lane: they are read as uniform (scalar-cache) loads.
+#define LZ_RR 4
 __global__ __launch_bounds__(LZ_T) void lz_reorth_norm(const LanczosState *sts, const LzInst *in)
 {
     const LanczosState *st = sts + blockIdx.y;
@@ -385,19 +389,39 @@ __global__ __launch_bounds__(LZ_T) void lz_reorth_norm(const LanczosState *sts, 
     const int nt = st->it + 2;
     const int64_t n = I.n;
     const LZ_CONST cplx *o = kp(I.ortho);
+    const LZ_GLOBAL cplx *V = gp(I.lvec);
+    LZ_GLOBAL cplx *HC = gp(I.HC);
     double s = 0;
-    LZ_ROWS(I, j)
-    {
-        cplx h = ldv(gp(I.HC) + j);
-#pragma unroll 4
-        for (int tt = 0; tt < nt; tt++) {
-            cplx x = ldv(gp(I.lvec) + (int64_t)tt * n + j);
-            const cplx ot = cmake(o[tt].re, o[tt].im);
-            h.re -= x.re * ot.re - x.im * ot.im;
-            h.im -= x.re * ot.im + x.im * ot.re;
+    for (int64_t base = (int64_t)blockIdx.x * LZ_T * LZ_RR + threadIdx.x; base < n;
+         base += (int64_t)I.nb * LZ_T * LZ_RR) {
+        cplx h[LZ_RR];
+#pragma unroll
+        for (int r = 0; r < LZ_RR; r++) {
+            const int64_t j = base + (int64_t)r * LZ_T;
+            h[r] = (j < n) ? ldv(HC + j) : cmake(0, 0);
         }
-        stv(gp(I.HC) + j, h);
-        s += h.re * h.re + h.im * h.im;
+#pragma unroll 2
+        for (int tt = 0; tt < nt; tt++) {
+            const cplx ot = cmake(o[tt].re, o[tt].im);
+            const LZ_GLOBAL cplx *vt = V + (int64_t)tt * n;
+#pragma unroll
+            for (int r = 0; r < LZ_RR; r++) {
+                const int64_t j = base + (int64_t)r * LZ_T;
+                if (j < n) {
+                    const cplx x = ldv(vt + j);
+                    h[r].re -= x.re * ot.re - x.im * ot.im;
+                    h[r].im -= x.re * ot.im + x.im * ot.re;
+                }
+            }
+        }
+#pragma unroll
+        for (int r = 0; r < LZ_RR; r++) {
+            const int64_t j = base + (int64_t)r * LZ_T;
+            if (j < n) {
+                stv(HC + j, h[r]);
+                s += h[r].re * h[r].re + h[r].im * h[r].im;
+            }
+        }
     }
     block_sum2(s, 0, I.partial);
 }
