@@ -93,7 +93,8 @@ def cpu_baseline(b, C, Ho, Hint, gpu_out, target_s: float = 12.0):
     }
 
 
-def batched_lanczos(n_inst: int = 64, lm: int = 120, seed: int = 64120, serial: bool = True):
+def batched_lanczos(n_inst: int = 64, lm: int = 120, seed: int = 64120, serial: bool = True,
+                    cpu_sample: bool = True):
     """64 Lieb-Liniger instances, (N, M) drawn from the seed over (6,9),
     (8,11), (10,13), g log-uniform in [0.1, 100], lm = 120 Lanczos steps from
     seeded random normalized complex128 start vectors already on the device.
@@ -136,8 +137,33 @@ def batched_lanczos(n_inst: int = 64, lm: int = 120, seed: int = 64120, serial: 
         for x in range(n_inst):
             E_s[x] = hams[x].ground(lm, starts[x], out=outs[x])[0]
         t_serial = time.perf_counter() - t
-    dE = float(np.abs(E_b - E_s).max())
+    dE = float(np.abs(E_b - E_s).max()) if serial else None
     counts = {f"({shapes[k][0]},{shapes[k][1]})": int((picks == k).sum()) for k in range(3)}
+    cpu = None
+    if serial and cpu_sample:
+        # the oracle's C/OpenMP restatement of lanczos + ground
+        # (groundStateLieb.c:217-471) on the host cores, one instance of each
+        # of the two smaller bases (the (10,13) tables take minutes to build on
+        # the host); compared with the same instances' GPU solves
+        import oracle
+        cpu = {"cores": oracle.num_threads(), "kind": "port", "per_basis": {}}
+        for k in (0, 1):
+            xs = [x for x in range(n_inst) if picks[x] == k]
+            if not xs:
+                continue
+            x = xs[0]
+            N, M = shapes[k]
+            o = oracle.Basis(N, M)
+            v0 = starts[x].numpy()
+            t = time.perf_counter()
+            E_cpu = o.ground(lm, v0, mcpi_amd.ll_ho(M), mcpi_amd.ll_hint(M, float(gs[x])))[0]
+            t_cpu = time.perf_counter() - t
+            mcpi_amd.synchronize()
+            t = time.perf_counter()
+            E_gpu = hams[x].ground(lm, starts[x], out=outs[x])[0]
+            t_gpu = time.perf_counter() - t
+            cpu["per_basis"][f"({N},{M})"] = {"cpu_s_per_gs": round(t_cpu, 4), "gpu_s_per_gs": round(t_gpu, 5),
+                                            "abs_dE0_cpu_vs_gpu": float(abs(E_cpu - E_gpu))}
     return {
         "workload": (f"{n_inst} Lieb-Liniger instances, (N,M) from the seed over (6,9)/(8,11)/(10,13) "
                      f"{counts}, g log-uniform [0.1,100], lm={lm}, full maps, complex128"),
@@ -149,6 +175,7 @@ def batched_lanczos(n_inst: int = 64, lm: int = 120, seed: int = 64120, serial: 
         "max_abs_dE0_batched_vs_per_instance": dE if serial else None,
         "all_steps_full": bool((steps_b == lm).all()),
         "basis_build_s_excluded": round(build_s, 3),
+        "cpu_oracle_sample": cpu,
     }
 
 
@@ -271,9 +298,10 @@ def main() -> int:
         H.apply(dC, dO)
     mcpi_amd.synchronize()
     t0 = time.perf_counter()
-    kern_ms = H.apply_timed(dC, dO, args.steps)     # exactly K launches, events on their stream
+    each = H.apply_timed_each(dC, dO, args.steps)   # exactly K launches, events on their stream
     mcpi_amd.synchronize()
     wall = time.perf_counter() - t0
+    kern_ms = float(each.sum() / args.steps)
     ms_step = wall / args.steps * 1e3
     value = B * args.steps / wall / 1e9
     achieved = B / (kern_ms * 1e-3) / 1e9
@@ -300,7 +328,7 @@ def main() -> int:
     batched = large = None
     if not (args.no_batched or args.profile):
         del dO
-        batched = batched_lanczos()
+        batched = batched_lanczos(cpu_sample=not args.no_cpu)
 
     cpu = None
     if not (args.no_cpu or args.profile):
@@ -341,6 +369,8 @@ def main() -> int:
             "device": name, "compute_units": cus,
             "hbm_fraction_wall": round(value / HBM_PEAK_GBS, 4),
             "kernel_ms_per_hv": round(kern_ms, 4),
+            "kernel_ms_p10_p50_p90": [round(float(np.percentile(each, q)), 4) for q in (10, 50, 90)],
+            "kernel_ms_total": round(float(each.sum()), 3),
             "basis_build_s": round(build_s, 3),
             "other_schedules": others,
             "batched_lanczos": batched,

@@ -236,6 +236,11 @@ mcpi_status mcpi_hip_applyHconf_dev(int N, int M, const int32_t *Map,
 mcpi_status mcpi_hip_applyH_timed(mcpi_ham h, const mcpi_cplx *C_dev,
                                   mcpi_cplx *out_dev, int reps, void *stream,
                                   double *avg_ms);
+/* the same, one duration per call: ms_each[reps] (events recorded between
+ * consecutive launches on the stream) */
+mcpi_status mcpi_hip_applyH_timed_each(mcpi_ham h, const mcpi_cplx *C_dev,
+                                       mcpi_cplx *out_dev, int reps, void *stream,
+                                       double *ms_each);
 
 /* density matrices of C (device vector): rho1 M*M, rho2 M^4, host outputs */
 mcpi_status mcpi_hip_rho1(mcpi_basis b, const mcpi_cplx *C_dev, mcpi_cplx *rho1);

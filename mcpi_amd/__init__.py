@@ -106,6 +106,7 @@ def lib():
         "mcpi_hip_lanczos_batched": (st, [ct.c_int, ct.POINTER(_vp), ct.POINTER(_vp), ct.c_int,
                                           _f64p, _i32p, ct.POINTER(_vp)]),
         "mcpi_hip_set_batched_kernel": (st, [ct.c_int]),
+        "mcpi_hip_applyH_timed_each": (st, [_vp, _vp, _vp, ct.c_int, _vp, _f64p]),
     }
     for name, (res, args) in sig.items():
         f = getattr(L, name)
@@ -330,6 +331,13 @@ class Hamiltonian:
         _check(lib().mcpi_hip_applyH_timed(self.handle, C.ptr, out.ptr, reps, stream, ct.byref(ms)),
                "applyH_timed")
         return ms.value
+
+    def apply_timed_each(self, C: DeviceArray, out: DeviceArray, reps: int, stream=None) -> np.ndarray:
+        """reps H.v launches; the duration (ms) of each, from HIP events."""
+        ms = np.zeros(reps, np.float64)
+        _check(lib().mcpi_hip_applyH_timed_each(self.handle, C.ptr, out.ptr, reps, stream, ms),
+               "applyH_timed_each")
+        return ms
 
     def __call__(self, C) -> np.ndarray:
         """H.v of a host vector (upload, apply, download)."""

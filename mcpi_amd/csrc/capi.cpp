@@ -191,6 +191,32 @@ mcpi_status mcpi_hip_applyH_timed(mcpi_ham h, const mcpi_cplx *C_dev, mcpi_cplx 
     return MCPI_OK;
 }
 
+mcpi_status mcpi_hip_applyH_timed_each(mcpi_ham h, const mcpi_cplx *C_dev, mcpi_cplx *out_dev, int reps,
+                                       void *stream, double *ms_each)
+{
+    MCPI_ARG(h && C_dev && out_dev && reps >= 1 && ms_each, "bad argument");
+    hipStream_t s = (hipStream_t)stream;
+    std::vector<hipEvent_t> ev(reps + 1, nullptr);
+    mcpi_status st = MCPI_OK;
+    for (int r = 0; r <= reps && st == MCPI_OK; r++)
+        if (hipEventCreate(&ev[r]) != hipSuccess) st = hip_fail(hipGetLastError(), "hipEventCreate", __FILE__, __LINE__);
+    for (int r = 0; r < reps && st == MCPI_OK; r++) {
+        if (hipEventRecord(ev[r], s) != hipSuccess) st = hip_fail(hipGetLastError(), "hipEventRecord", __FILE__, __LINE__);
+        if (st == MCPI_OK) st = launch_apply(h, (const cplx *)C_dev, (cplx *)out_dev, s);
+    }
+    if (st == MCPI_OK && (hipEventRecord(ev[reps], s) != hipSuccess || hipEventSynchronize(ev[reps]) != hipSuccess))
+        st = hip_fail(hipGetLastError(), "event", __FILE__, __LINE__);
+    for (int r = 0; r < reps && st == MCPI_OK; r++) {
+        float ms = 0;
+        if (hipEventElapsedTime(&ms, ev[r], ev[r + 1]) != hipSuccess)
+            st = hip_fail(hipGetLastError(), "hipEventElapsedTime", __FILE__, __LINE__);
+        ms_each[r] = ms;
+    }
+    for (hipEvent_t e : ev)
+        if (e) (void)hipEventDestroy(e);
+    return st;
+}
+
 mcpi_status mcpi_hip_applyHconf_dev(int N, int M, const int32_t *Map, const int32_t *MapOT,
                                     const int32_t *MapTT, const int64_t *strideOT, const int64_t *strideTT,
                                     const int32_t *IF, const mcpi_cplx *C, const mcpi_cplx *Ho,
