@@ -44,7 +44,7 @@ __device__ inline void chunk_list_multi(const int32_t *__restrict__ slot, int na
             const int q = e % M, l = e / M;
             if (DMAP) {
                 const int tt = l * (l + 1) / 2 + q;
-                j[u] = slot[(int64_t)(tt >> 2) * nact4 + (tt & 3)];
+                j[u] = ld_nt(slot + (int64_t)(tt >> 2) * nact4 + (tt & 3));
             } else {
                 j[u] = (int64_t)(uint32_t)(base - dl<M>(Dl, lane, T, 0, q) - dl<M>(Dl, lane, T, 1, l));
             }
@@ -162,7 +162,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void k
             const int l = __ffs(m) - 1;
             m &= m - 1;
             const int kl = k + M * l;
-            const int64_t j = act ? (int64_t)b.Map[i + (int64_t)kl * nc] : i;
+            const int64_t j = act ? (int64_t)ld_nt(b.Map + i + (int64_t)kl * nc) : i;
             const double x = fk * sq[occ<M>(r, l) + 1];
 #pragma unroll
             for (int v = 0; v < NV; v++) {
