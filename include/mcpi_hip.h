@@ -191,7 +191,9 @@ mcpi_status mcpi_hip_basis_copy_tt(mcpi_basis b, int64_t *strideTT, int32_t *Map
  * poff[w] and following each other: slot[g][a][u] = target of entry
  * t = 4g + u of the a-th active row (t = l(l+1)/2 + q, q <= l; entries
  * t >= M(M+1)/2 hold the row's own index).  The target of (l, q) is the
- * reference map's entry l + q*M of that chunk. */
+ * reference map's entry l + q*M of that chunk, except where q or l is a
+ * removed orbital of the jump (k, or k and s): those entries are no
+ * double-jump term and hold the row's own index. */
 mcpi_status mcpi_hip_basis_copy_range(mcpi_basis b, int which, int64_t offset,
                                       int64_t count, void *dst);
 /* rank / unrank of many states on the GPU: v is n*M ints (host) */
@@ -215,7 +217,10 @@ mcpi_status mcpi_hip_ham_set_skip_zero(mcpi_ham h, int on);
  * targets of the non-zero double-jump terms only, built once per basis and
  * zero pattern on first use and kept with the basis (sparse Hamiltonians
  * with Map stored; falls back to 2 otherwise, or when device memory is
- * short, or on a stream being captured before the tables exist) */
+ * short, or on a stream being captured before the tables exist),
+ * 5 = every term, with the double jumps of each block of 256 rows
+ * compacted into full wavefronts before their targets are read from the
+ * wave-packed tables (full maps; falls back to 2 otherwise) */
 mcpi_status mcpi_hip_ham_set_schedule(mcpi_ham h, int schedule);
 /* schedule in use (after autotuning; 0 before the first H.v) */
 mcpi_status mcpi_hip_ham_get_schedule(mcpi_ham h, int *schedule);

@@ -314,7 +314,9 @@ class Hamiltonian:
     def set_schedule(self, schedule: int) -> None:
         """0 auto-tuned, 1 generic, 2 specialised reading all maps, 3
         specialised ranking double jumps on the fly, 4 specialised reading
-        the pattern tables of the zero pattern (zero-skipping)."""
+        the pattern tables of the zero pattern (zero-skipping), 5 every
+        term with each block's double jumps compacted into full wavefronts
+        (full maps)."""
         _check(lib().mcpi_hip_ham_set_schedule(self.handle, int(schedule)), "schedule")
 
     @property

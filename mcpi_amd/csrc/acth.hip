@@ -393,6 +393,13 @@ mcpi_status launch_apply_multi(const BasisDev &b, const HamCoefDev &c0, int sche
 mcpi_status launch_apply_raw(const BasisDev &b, int maps, const HamCoefDev &c, int skip_zero,
                              int schedule, const cplx *C, cplx *out, hipStream_t st)
 {
+    if (schedule == 5) {
+        if (maps == MCPI_MAPS_FULL) {
+            mcpi_status s = launch_comp(b, c, C, out, st);
+            if (s != MCPI_ERR_STATE) return s;
+        }
+        schedule = 2;
+    }
     if (schedule == 2 || schedule == 3) {
         mcpi_status s = launch_fast(b, maps, c, skip_zero && !c.dense, schedule == 3, C, out, st);
         if (s != MCPI_ERR_STATE) return s;
@@ -453,11 +460,12 @@ static mcpi_status autotune(mcpi_ham h, const cplx *C, cplx *out, hipStream_t st
 {
     const BasisDev &b = h->basis->d;
     const int maps = h->basis->maps;
-    int cands[4], n = 0;
+    int cands[5], n = 0;
     cands[n++] = 1;
     if (has_fast_kernel(b.N, b.M)) {
         cands[n++] = 2;
         if (maps == MCPI_MAPS_FULL) cands[n++] = 3;
+        if (maps == MCPI_MAPS_FULL && b.packOT && b.packTT) cands[n++] = 5;
         if (pattern_schedule_ok(h) && basis_pattern(h->basis, h->c, true, st)) cands[n++] = 4;
     }
     int best = cands[n - 1];

@@ -188,8 +188,10 @@ __device__ inline uint32_t dl(const uint32_t *Dl, int lane, int T, int a, int x)
 
 // ------------------------------------------------------- chunk bodies --
 
-// every entry q <= l of one chunk (coefficients of excluded entries are
-// zero; their targets are valid states, so no branch is needed).  Row l:
+// every entry q <= l of one chunk (entries whose q or l is a removed orbital
+// are no rule term: their folded coefficients are zero and their targets
+// valid states, so no branch is needed; the packed tables hold the row's
+// own index for them, a coalesced cached gather).  Row l:
 // sum_{q<l} coef r1[q] C[j] is scaled once by r1[l]; q == l uses r2[l].
 // Targets ranked from the difference tables.
 template <int M, bool RC>

@@ -359,9 +359,14 @@ __global__ void k_packed_fill(mcpi::BasisDev b)
             if (act) {
                 int t = 0;
                 for (int l = 0; l < M; l++)
-                    for (int q = 0; q <= l; q++, t++)
+                    for (int q = 0; q <= l; q++, t++) {
+                        // an entry adding at a removed orbital is no double-jump
+                        // term (its folded coefficient is zero): the row's own
+                        // index keeps its gather coalesced and cached
+                        const bool excl = q == k || q == s || l == k || l == s;
                         pack[base + ((int64_t)(t >> 2) * nact + my) * 4 + (t & 3)] =
-                            (int32_t)rank_moved(P, Bt, N, M, b.nc, k, s, l, q);
+                            excl ? (int32_t)i : (int32_t)rank_moved(P, Bt, N, M, b.nc, k, s, l, q);
+                    }
                 for (; t < E; t++) pack[base + ((int64_t)(t >> 2) * nact + my) * 4 + (t & 3)] = (int32_t)i;
             }
             (void)MH;

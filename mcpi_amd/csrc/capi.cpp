@@ -152,7 +152,7 @@ mcpi_status mcpi_hip_ham_set_skip_zero(mcpi_ham h, int on)
 mcpi_status mcpi_hip_ham_set_schedule(mcpi_ham h, int schedule)
 {
     MCPI_ARG(h, "null handle");
-    MCPI_ARG(schedule >= 0 && schedule <= 4, "schedule must be 0 (auto), 1, 2, 3 or 4");
+    MCPI_ARG(schedule >= 0 && schedule <= 5, "schedule must be 0 (auto), 1, 2, 3, 4 or 5");
     h->schedule = schedule;
     return MCPI_OK;
 }

@@ -220,8 +220,12 @@ mcpi_status launch_apply_batch(const BasisDev &b, int maps, const HamCoefDev &c0
                                size_t cstride, const cplx *C0, cplx *out0, int64_t vstride, hipStream_t st);
 mcpi_status launch_apply_multi(const BasisDev &b, const HamCoefDev &c0, int sched, int nbatch, size_t cstride,
                                const cplx *C0, cplx *out0, int64_t vstride, hipStream_t st);
+// schedule 5 kernel (acth_comp.hip); MCPI_ERR_STATE when not applicable
+mcpi_status launch_comp(const BasisDev &b, const HamCoefDev &c, const cplx *C, cplx *out, hipStream_t st);
 // schedules: 1 generic thread-per-row kernel, 2 specialised kernel reading
-// every stored map, 3 specialised kernel ranking double jumps on the fly
+// every stored map, 3 specialised kernel ranking double jumps on the fly,
+// 5 every term with the double jumps of a block compacted into full
+// wavefronts (Map and packed tables; 2 when the basis has no packed tables)
 mcpi_status launch_apply_raw(const BasisDev &b, int maps, const HamCoefDev &c, int skip_zero,
                              int schedule, const cplx *C, cplx *out, hipStream_t st);
 mcpi_status build_coefficients(int M, const cplx *Ho, const cplx *Hint, HamCoefDev *c, void **block,

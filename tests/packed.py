@@ -1,7 +1,10 @@
 """Test-side restatement of the wave-packed double-jump layout (basis.hip,
 k_packed_counts / k_packed_fill): built from the reference-layout maps
 (configurationsMap.h:421-762 chunk order) so that the GPU tables can be
-compared entry by entry."""
+compared entry by entry.  Entries whose added orbital q or l is one of the
+chunk's removed orbitals k, s are no double-jump term (the rules fold them
+into zero coefficients) and hold the row's own index instead of the map's
+target."""
 import numpy as np
 
 
@@ -33,7 +36,8 @@ def expected_wave(IF, M, rows, Map, stride, stride_base, tt: bool):
         for a, ri in enumerate(idx):
             r = rows[ri]
             beg = stride[r - stride_base] + ordinal[ri] * M * M
-            vals = [Map[beg + l + q * M] for (l, q) in tri] + [r] * (E - len(tri))
+            # entries adding at a removed orbital (q or l in {k, s}) hold r
+            vals = [r if {q, l} & {k, s} else Map[beg + l + q * M] for (l, q) in tri] + [r] * (E - len(tri))
             slot[:, a, :] = np.asarray(vals).reshape(E // 4, 4)
             ordinal[ri] += 1
         out.append(slot.reshape(-1))

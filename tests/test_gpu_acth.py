@@ -23,7 +23,9 @@ def schedules(M, maps):
     """1 generic; 2 specialised (reads every stored map); 3 specialised,
     double jumps ranked on the fly (differs from 2 only with full maps);
     4 specialised with the pattern tables of the zero pattern (zero-skipping
-    on a sparse Hamiltonian with Map stored; otherwise it runs as 2)."""
+    on a sparse Hamiltonian with Map stored; otherwise it runs as 2);
+    5 every term with each block's double jumps compacted into full
+    wavefronts (full maps; otherwise it runs as 2)."""
     out = [1]
     if 3 <= M <= 13:
         out.append(2)
@@ -31,6 +33,7 @@ def schedules(M, maps):
             out.append(3)
         if maps != "none":
             out.append(4)
+        out.append(5)
     return out
 
 
@@ -171,7 +174,7 @@ def test_n12_m12_generic_sampled_rows(oracle_lib):
     out = H(C)
     rows = np.concatenate([[0, b.nc - 1], rng.integers(0, b.nc, 1500)])
     assert_close(out[rows], oracle.acth_hash_rows(N, M, rows, C, Ho, Hint), 1e-12, "rows")
-    for sched in (1, 2, 3):
+    for sched in (1, 2, 3, 5):
         H2 = mcpi_amd.Hamiltonian(b, Ho, Hint, skip_zero=False, schedule=sched)
         assert_close(H2(C), out, 1e-12, f"schedule {sched}")
 
