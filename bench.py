@@ -234,8 +234,9 @@ def main() -> int:
     ap.add_argument("--profile", action="store_true",
                     help="time only the generic H.v kernel (no CPU baseline, no zero-skipping run)")
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
-    ap.add_argument("--schedule", type=int, default=2,
-                    help="H.v schedule timed as the headline (2: reads every stored map)")
+    ap.add_argument("--schedule", type=int, default=5,
+                    help="H.v schedule timed as the headline (5: every term, every stored map "
+                         "entry read, double jumps compacted into full wavefronts)")
     ap.add_argument("--skip-zero", action="store_true",
                     help="time the zero-skipping path instead (not the headline)")
     ap.add_argument("--no-batched", action="store_true", help="skip the batched Lanczos comparison")
@@ -291,8 +292,9 @@ def main() -> int:
     dO = mcpi_amd.DeviceArray(b.nc)
     B = algorithmic_bytes(b.nc, M_, b.ot_len, b.tt_len)
 
-    # the timed path: every term evaluated, every map entry read (schedule 2:
-    # kernel specialised on M that loads each row's map chunks)
+    # the timed path: every term evaluated, every map entry read (schedule 5:
+    # kernel specialised on M; the double jumps of each block of 256 rows are
+    # compacted into full wavefronts before their packed targets are read)
     H = mcpi_amd.Hamiltonian(b, Ho, Hint, skip_zero=args.skip_zero, schedule=args.schedule)
     for _ in range(args.warmup):
         H.apply(dC, dO)
@@ -308,13 +310,15 @@ def main() -> int:
     gpu_out = dO.numpy()
 
     # other schedules of the same H.v (same result), for reference:
-    # generic thread-per-row kernel, double jumps ranked instead of read, and
-    # the autotuned Lieb-Liniger-aware path that skips exact-zero terms
+    # generic thread-per-row kernel, the thread-per-row specialised kernel
+    # reading the same tables, double jumps ranked instead of read, and the
+    # autotuned Lieb-Liniger-aware path that skips exact-zero terms
     others = {}
     agree = 0.0
     if not args.profile:
         reps = max(10, args.steps // 10)
         for label, kw in (("generic_rows_kernel", dict(skip_zero=False, schedule=1)),
+                          ("packed_rows_per_lane", dict(skip_zero=False, schedule=2)),
                           ("ranked_doubles", dict(skip_zero=False, schedule=3)),
                           ("skip_zero_autotuned", dict(skip_zero=True, schedule=0))):
             Hx = mcpi_amd.Hamiltonian(b, Ho, Hint, **kw)
