@@ -293,16 +293,20 @@ __device__ inline cplx chunk_list(const int32_t *__restrict__ slot, int nact4, c
     return part;
 }
 
-// the listed entries of one jump read from the pattern tables: four
-// targets per 16-byte load (slot group g = entries 4g .. 4g+3 of the list)
+// the listed entries of one jump read from the pattern tables: entry t of
+// this lane at slot + t * nact (entry-major slots), four entries at a time
+// with their target loads ahead of the gathers; past n the lane's own row
+// stands in (coefficient zero)
 template <int M, bool RC>
-__device__ inline cplx chunk_pat(const int32_t *__restrict__ slot, int nact4, const cplx *__restrict__ C,
+__device__ inline cplx chunk_pat(const int32_t *__restrict__ slot, int nact, const cplx *__restrict__ C,
                                  const RowState<M> &r, const double *sq, const int32_t *ents, const cplx *coefs,
-                                 int n)
+                                 int n, int32_t self)
 {
     cplx part = cmake(0, 0);
     for (int t = 0; t < n; t += 4) {
-        const Piece4 g = *reinterpret_cast<const Piece4 *>(slot + (int64_t)(t >> 2) * nact4);
+        Piece4 g;
+#pragma unroll
+        for (int u = 0; u < 4; u++) g.x[u] = t + u < n ? ld_nt(slot + (int64_t)(t + u) * nact) : self;
         double x[4];
         cplx cf[4];
 #pragma unroll
@@ -445,10 +449,10 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(SKIP ? 4 : 
             const bool act = vk >= 2;
             const uint64_t m = __ballot(act);
             const int nact = __popcll(m);
-            const int32_t *slot = b.patOT + pb + 4 * __popcll(m & below);
-            pb += (int64_t)nact * ((n_nz + 3) & ~3);
+            const int32_t *slot = b.patOT + pb + __popcll(m & below);
+            pb += (int64_t)nact * n_nz;
             if (!act) continue;
-            const cplx part = chunk_pat<M, RC>(slot, 4 * nact, C, r, sq, c.nz_e + beg, c.nz_c + beg, n_nz);
+            const cplx part = chunk_pat<M, RC>(slot, nact, C, r, sq, c.nz_e + beg, c.nz_c + beg, n_nz, (int32_t)i);
             const double f = sq[vk] * sq[vk - 1];
             acc.re = fma(f, part.re, acc.re);
             acc.im = fma(f, part.im, acc.im);
@@ -464,10 +468,11 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(SKIP ? 4 : 
                 const bool act = vk >= 1 && vs >= 1;
                 const uint64_t m = __ballot(act);
                 const int nact = __popcll(m);
-                const int32_t *slot = b.patTT + pb + 4 * __popcll(m & below);
-                pb += (int64_t)nact * ((n_nz + 3) & ~3);
+                const int32_t *slot = b.patTT + pb + __popcll(m & below);
+                pb += (int64_t)nact * n_nz;
                 if (!act) continue;
-                const cplx part = chunk_pat<M, RC>(slot, 4 * nact, C, r, sq, c.nz_e + beg, c.nz_c + beg, n_nz);
+                const cplx part =
+                    chunk_pat<M, RC>(slot, nact, C, r, sq, c.nz_e + beg, c.nz_c + beg, n_nz, (int32_t)i);
                 const double f = sq[vk] * sq[vs];
                 acc.re = fma(f, part.re, acc.re);
                 acc.im = fma(f, part.im, acc.im);

@@ -65,16 +65,19 @@ __device__ inline void chunk_list_multi(const int32_t *__restrict__ slot, int na
     }
 }
 
-// the listed entries of one jump from the pattern tables (four targets per
-// 16-byte load), for NV instances
+// the listed entries of one jump from the pattern tables (entry t of this
+// lane at slot + t * nact), for NV instances; past n the lane's own row
+// stands in (coefficient zero)
 template <int M, int NV, bool RC>
-__device__ inline void chunk_pat_multi(const int32_t *__restrict__ slot, int nact4, const RowState<M> &r,
+__device__ inline void chunk_pat_multi(const int32_t *__restrict__ slot, int nact, const RowState<M> &r,
                                        const double *sq, const int32_t *ents, const cplx *coefs0, size_t cstride,
-                                       const cplx *__restrict__ C0, int64_t vstride, int nv, int n,
+                                       const cplx *__restrict__ C0, int64_t vstride, int nv, int n, int32_t self,
                                        cplx (&part)[NV])
 {
     for (int t = 0; t < n; t += 4) {
-        const Piece4 g = *reinterpret_cast<const Piece4 *>(slot + (int64_t)(t >> 2) * nact4);
+        Piece4 g;
+#pragma unroll
+        for (int u = 0; u < 4; u++) g.x[u] = t + u < n ? ld_nt(slot + (int64_t)(t + u) * nact) : self;
         double x[4];
 #pragma unroll
         for (int u = 0; u < 4; u++) {
@@ -195,14 +198,14 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void k
             const bool act = vk >= 2;
             const uint64_t m = __ballot(act);
             const int nact = __popcll(m);
-            const int32_t *slot = b.patOT + pb + 4 * __popcll(m & below);
-            pb += (int64_t)nact * ((n_nz + 3) & ~3);
+            const int32_t *slot = b.patOT + pb + __popcll(m & below);
+            pb += (int64_t)nact * n_nz;
             if (!act) continue;
             cplx part[NV];
 #pragma unroll
             for (int v = 0; v < NV; v++) part[v] = cmake(0, 0);
-            chunk_pat_multi<M, NV, RC>(slot, 4 * nact, r, sq, c0.nz_e + beg, nzc + beg, cstride, C, vstride, nv,
-                                       n_nz, part);
+            chunk_pat_multi<M, NV, RC>(slot, nact, r, sq, c0.nz_e + beg, nzc + beg, cstride, C, vstride, nv,
+                                       n_nz, (int32_t)i, part);
             const double f = sq[vk] * sq[vk - 1];
 #pragma unroll
             for (int v = 0; v < NV; v++) {
@@ -221,14 +224,14 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void k
                 const bool act = vk >= 1 && vs >= 1;
                 const uint64_t m = __ballot(act);
                 const int nact = __popcll(m);
-                const int32_t *slot = b.patTT + pb + 4 * __popcll(m & below);
-                pb += (int64_t)nact * ((n_nz + 3) & ~3);
+                const int32_t *slot = b.patTT + pb + __popcll(m & below);
+                pb += (int64_t)nact * n_nz;
                 if (!act) continue;
                 cplx part[NV];
 #pragma unroll
                 for (int v = 0; v < NV; v++) part[v] = cmake(0, 0);
-                chunk_pat_multi<M, NV, RC>(slot, 4 * nact, r, sq, c0.nz_e + beg, nzc + beg, cstride, C, vstride,
-                                           nv, n_nz, part);
+                chunk_pat_multi<M, NV, RC>(slot, nact, r, sq, c0.nz_e + beg, nzc + beg, cstride, C, vstride,
+                                           nv, n_nz, (int32_t)i, part);
                 const double f = sq[vk] * sq[vs];
 #pragma unroll
                 for (int v = 0; v < NV; v++) {

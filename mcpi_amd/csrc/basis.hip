@@ -380,12 +380,11 @@ __global__ void k_packed_fill(mcpi::BasisDev b)
 // For a Hamiltonian whose folded double-jump coefficients are mostly zero
 // (Lieb-Liniger keeps about 3 of the 78 entries of a chunk), the zero-
 // skipping H.v only needs the targets of the listed entries.  These tables
-// hold exactly those, in the wave-packed slot layout: for wavefront w and
-// each jump with a non-empty list that is active in at least one lane, one
-// slot [g][a][4] = target of list entry t = 4g + u of the a-th active lane
-// (t < n; padding entries hold the row's own index), n rounded up to 4.
-
-__device__ inline int round4(int n) { return (n + 3) & ~3; }
+// hold exactly those: for wavefront w and each jump with a non-empty list of
+// n entries that is active in at least one lane, one slot [t][a] = target
+// of list entry t of the a-th active lane (entry-major: the wavefront reads
+// entry t of all its active lanes as one contiguous run; no padding, so the
+// tables hold 4 bytes per listed term and row).
 
 __global__ void k_pattern_counts(mcpi::BasisDev b, HamCoefDev c, int32_t *__restrict__ cnt_ot,
                                  int32_t *__restrict__ cnt_tt)
@@ -404,13 +403,13 @@ __global__ void k_pattern_counts(mcpi::BasisDev b, HamCoefDev c, int32_t *__rest
     int64_t ot = 0, tt = 0;
     for (int k = 0; k < M; k++) {
         const int n = c.ot_off[k + 1] - c.ot_off[k];
-        if (n) ot += (int64_t)__popcll(__ballot(valid && ((m2 >> k) & 1u))) * round4(n);
+        if (n) ot += (int64_t)__popcll(__ballot(valid && ((m2 >> k) & 1u))) * n;
     }
     int p = 0;
     for (int k = 0; k < M; k++)
         for (int s = k + 1; s < M; s++, p++) {
             const int n = c.tt_off[p + 1] - c.tt_off[p];
-            if (n) tt += (int64_t)__popcll(__ballot(valid && ((m1 >> k) & 1u) && ((m1 >> s) & 1u))) * round4(n);
+            if (n) tt += (int64_t)__popcll(__ballot(valid && ((m1 >> k) & 1u) && ((m1 >> s) & 1u))) * n;
         }
     if ((threadIdx.x & 63) == 0 && w < nw) {
         cnt_ot[w] = (int32_t)ot;
@@ -453,17 +452,13 @@ __global__ void k_pattern_fill(mcpi::BasisDev b, HamCoefDev c, const int64_t *__
             const bool act = valid && (TT ? (((m1 >> k) & 1u) && ((m1 >> s) & 1u)) : ((m2 >> k) & 1u));
             const uint64_t m = __ballot(act);
             if (m == 0) continue;
-            const int nact = __popcll(m), my = __popcll(m & below), n4 = round4(n);
+            const int nact = __popcll(m), my = __popcll(m & below);
             if (act)
-                for (int t = 0; t < n4; t++) {
-                    int32_t val = (int32_t)i;
-                    if (t < n) {
-                        const int e = c.nz_e[beg + t];
-                        val = (int32_t)rank_moved(P, Bt, N, M, b.nc, k, s, e / M, e % M);
-                    }
-                    tab[base + ((int64_t)(t >> 2) * nact + my) * 4 + (t & 3)] = val;
+                for (int t = 0; t < n; t++) {
+                    const int e = c.nz_e[beg + t];
+                    tab[base + (int64_t)t * nact + my] = (int32_t)rank_moved(P, Bt, N, M, b.nc, k, s, e / M, e % M);
                 }
-            base += (int64_t)nact * n4;
+            base += (int64_t)nact * n;
         }
     }
 }
