@@ -76,6 +76,7 @@ __device__ inline cplx row_part(const BasisDev &b, const HamCoefDev &c, const cp
     for (int k = 0; k < M; k++) {
         const int vk = occ<M>(r, k);
         const bool act = vk >= 1;
+        if (__ballot(act) == 0) continue;   // orbital k empty in every row of the wavefront
         const double fk = act ? sq[vk] : 0.0;
 #pragma unroll
         for (int l = 0; l < M; l++) {

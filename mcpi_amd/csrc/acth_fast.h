@@ -390,6 +390,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(SKIP ? 4 : 
         if (SKIP && lmask == 0) continue;
         const int vk = occ<M>(r, k);
         const bool act = vk >= 1;
+        if (!SKIP && __ballot(act) == 0) continue;   // orbital k empty in every row of the wavefront
         uint32_t D0[M], D1[M], tot = 0;
         if (!MAP1) build_d<M, false>(r, Bt, N, k, -1, D0, D1, tot);
         if (!MAP1 && SKIP) store_d<M>(D0, D1, Dl, lane, T);
