@@ -186,14 +186,15 @@ mcpi_status mcpi_hip_basis_copy_tt(mcpi_basis b, int64_t *strideTT, int32_t *Map
  * strides and packed offsets are int64, the others int32).
  * Wave-packed tables (full maps): wavefront w = rows 64w .. 64w+63; for each
  * jump active in at least one of its rows (OT: k with v_k >= 2, ascending;
- * TT: pairs k < s both occupied, lexicographic) one slot of
- * E = 4*ceil(M(M+1)/8) entries per active row, starting at int offset
- * poff[w] and following each other: slot[g][a][u] = target of entry
- * t = 4g + u of the a-th active row (t = l(l+1)/2 + q, q <= l; entries
- * t >= M(M+1)/2 hold the row's own index).  The target of (l, q) is the
- * reference map's entry l + q*M of that chunk, except where q or l is a
- * removed orbital of the jump (k, or k and s): those entries are no
- * double-jump term and hold the row's own index. */
+ * TT: pairs k < s both occupied, lexicographic) one slot of E entries per
+ * active row, starting at int offset poff[w] and following each other:
+ * slot[g][a][u] = target of entry t = 4g + u of the a-th active row.  A slot
+ * holds the entries q <= l of the chunk with neither q nor l a removed
+ * orbital of the jump (k, or k and s; the others are no double-jump term),
+ * numbered over the MP = M-1 (OT) or M-2 (TT) remaining orbitals
+ * o' = o - [k < o] - [s < o]: t = l'(l'+1)/2 + q'; E = MP(MP+1)/2 rounded
+ * up to a multiple of four, the padding holding the row's own index.  The
+ * target of (q, l) is the reference map's entry l + q*M of that chunk. */
 mcpi_status mcpi_hip_basis_copy_range(mcpi_basis b, int which, int64_t offset,
                                       int64_t count, void *dst);
 /* rank / unrank of many states on the GPU: v is n*M ints (host) */

@@ -85,6 +85,27 @@ mcpi_status build_coefficients(int M, const cplx *Ho, const cplx *Hint, HamCoefD
                     nzt[p * M2 + e] = nzc(a);
                 }
         }
+    // the same coefficients in packed-slot order (entries (q, l) outside the
+    // removed orbitals, renumbered without them; padding zero)
+    const int EO = packed_entries(M, false), ET = packed_entries(M, true);
+    const int MPO = packed_orbitals(M, false), MPT = packed_orbitals(M, true);
+    std::vector<cplx> otc((size_t)M * EO, cmake(0, 0)), ttc((size_t)std::max(P, 1) * ET, cmake(0, 0));
+    for (int k = 0; k < M; k++) {
+        int t = 0;
+        for (int lp = 0; lp < MPO; lp++)
+            for (int qp = 0; qp <= lp; qp++, t++)
+                otc[(size_t)k * EO + t] =
+                    ot[k * M2 + packed_orbital(k, k, qp) + (int64_t)M * packed_orbital(k, k, lp)];
+    }
+    for (int k = 0; k < M; k++)
+        for (int s = k + 1; s < M; s++) {
+            const int p = pair_index(k, s, M);
+            int t = 0;
+            for (int lp = 0; lp < MPT; lp++)
+                for (int qp = 0; qp <= lp; qp++, t++)
+                    ttc[(size_t)p * ET + t] =
+                        tt[p * M2 + packed_orbital(k, s, qp) + (int64_t)M * packed_orbital(k, s, lp)];
+        }
     // zero-skipping lists: per-k single-jump masks and, per OT / TT chunk,
     // the entries with a non-zero folded coefficient
     std::vector<uint32_t> m1j(M, 0);
@@ -111,7 +132,8 @@ mcpi_status build_coefficients(int M, const cplx *Ho, const cplx *Hint, HamCoefD
     nz_e.resize((size_t)(M + std::max(P, 1)) * M2 + 1, 0);
     nz_c.resize(nz_e.size(), cmake(0, 0));
     // one device block: complex tables first (16-byte aligned), then integers
-    size_t nC = dk.size() + d1.size() + d2.size() + A0.size() + A.size() + ot.size() + tt.size() + nz_c.size();
+    size_t nC = dk.size() + d1.size() + d2.size() + A0.size() + A.size() + ot.size() + tt.size() + nz_c.size() +
+                otc.size() + ttc.size();
     size_t nB = nz1.size() + nzo.size() + nzt.size() + 4 * (m1j.size() + ot_off.size() + tt_off.size() +
                                                            nz_e.size()) + 64;
     size_t bytes = (nC * sizeof(cplx) + nB + 64 + 255) & ~(size_t)255;
@@ -128,6 +150,8 @@ mcpi_status build_coefficients(int M, const cplx *Ho, const cplx *Hint, HamCoefD
     size_t o_ot = put(ot.data(), ot.size() * sizeof(cplx));
     size_t o_tt = put(tt.data(), tt.size() * sizeof(cplx));
     size_t o_nc = put(nz_c.data(), nz_c.size() * sizeof(cplx));
+    size_t o_otc = put(otc.data(), otc.size() * sizeof(cplx));
+    size_t o_ttc = put(ttc.data(), ttc.size() * sizeof(cplx));
     size_t o_m1 = put(m1j.data(), m1j.size() * 4);
     size_t o_oo = put(ot_off.data(), ot_off.size() * 4);
     size_t o_to = put(tt_off.data(), tt_off.size() * 4);
@@ -147,6 +171,8 @@ mcpi_status build_coefficients(int M, const cplx *Ho, const cplx *Hint, HamCoefD
     c->A = (cplx *)(dev + o_A);
     c->ot = (cplx *)(dev + o_ot);
     c->tt = (cplx *)(dev + o_tt);
+    c->otc = (cplx *)(dev + o_otc);
+    c->ttc = (cplx *)(dev + o_ttc);
     c->nz_1j = (uint8_t *)(dev + o_n1);
     c->nz_ot = (uint8_t *)(dev + o_no);
     c->nz_tt = (uint8_t *)(dev + o_nt);

@@ -33,7 +33,6 @@ namespace {
 // batch at M = 12)
 #define MCPI_CONST __attribute__((address_space(4)))
 typedef const MCPI_CONST cplx *coef_ptr;
-__device__ inline cplx ldk(coef_ptr p, int x) { return cmake(p[x].re, p[x].im); }   // field by field
 
 // wavefronts per block: 4 (43.6 KB of LDS at M = 12, two blocks per CU);
 // 8 needs 119 KB, one block per CU, and was slower (3.81 ms instead of 3.37
@@ -96,46 +95,6 @@ __device__ inline cplx row_part(const BasisDev &b, const HamCoefDev &c, const cp
     return acc;
 }
 
-// chunk sum of one (row, jump) pair (chunk_packed of acth_fast.h) with the
-// row's sqrt(v + 1) read from the LDS table where it is used instead of held
-// in registers: the compacted kernel carries more state than the
-// thread-per-row one and would otherwise spill
-// (slot = tab + off: a block-uniform base and a 32-bit lane offset, so the
-// group loads address with one register per lane)
-template <int M, bool RC>
-__device__ inline cplx chunk_packed_c(const int32_t *__restrict__ tab, uint32_t off, uint32_t nact4,
-                                      const cplx *__restrict__ C, const uint64_t (&pk)[2], const double *sq,
-                                      coef_ptr coef)
-{
-    constexpr int G = packed_entries(M) / 4;
-    int32_t idx[4 * G];
-#pragma unroll
-    for (int g = 0; g < G; g++) {
-        const Piece4 pc = ld_group_nt(tab + (off + (uint32_t)g * nact4));
-#pragma unroll
-        for (int u = 0; u < 4; u++) idx[4 * g + u] = pc.x[u];
-    }
-    __builtin_amdgcn_sched_barrier(0);
-    cplx part = cmake(0, 0);
-#pragma unroll
-    for (int l = 0; l < M; l++) {
-        cplx pl = cmake(0, 0);
-#pragma unroll
-        for (int q = 0; q <= l; q++) {
-            const int t = l * (l + 1) / 2 + q;
-            const cplx cj = ld_c(C + (int64_t)idx[t]);
-            const int vq = (int)((q < 8 ? (pk[0] >> (8 * q)) : (pk[1] >> (8 * (q - 8)))) & 0xff);
-            if (q < l) pl = term<RC>(ldk(coef, q + l * M), sq[vq + 1], cj, pl);
-            else part = term<RC>(ldk(coef, q + l * M), sq[vq + 1] * sq[vq + 2], cj, part);
-        }
-        const int vl = (int)((l < 8 ? (pk[0] >> (8 * l)) : (pk[1] >> (8 * (l - 8)))) & 0xff);
-        const double rl = sq[vl + 1];
-        part.re = fma(rl, pl.re, part.re);
-        part.im = fma(rl, pl.im, part.im);
-    }
-    return part;
-}
-
 // the batches of one double jump J (OT when TT is false: k == s) dealt to
 // this wavefront; bctr counts the block's batches so far
 template <int M, bool RC, bool TT>
@@ -180,7 +139,9 @@ __device__ inline void comp_jump(const int32_t *__restrict__ tab, const cplx *__
         const int rho = w * 64 + rows[w * 64 + aw];
         const uint64_t rp[2] = {pk[rho], pk[CR + rho]};
         const uint32_t off = (uint32_t)(wb[w] + pbl[w] + 4 * aw);
-        const cplx part = chunk_packed_c<M, RC>(tab, off, 4u * (uint32_t)nw, C, rp, sq, coef);
+        uint64_t pkc[2];
+        compress_occ(rp, k, s, pkc);
+        const cplx part = chunk_packed<M, RC, TT>(tab, off, 4u * (uint32_t)nw, C, pkc, sq, coef);
         const int vk = (int)((k < 8 ? (rp[0] >> (8 * k)) : (rp[1] >> (8 * (k - 8)))) & 0xff);
         const int vs = (int)((s < 8 ? (rp[0] >> (8 * s)) : (rp[1] >> (8 * (s - 8)))) & 0xff);
         const double f = TT ? sq[vk] * sq[vs] : sq[vk] * sq[vk - 1];
@@ -196,8 +157,7 @@ __global__ __launch_bounds__(CR) __attribute__((amdgpu_waves_per_eu(2))) void k_
     BasisDev b, HamCoefDev c, const cplx *__restrict__ C, cplx *__restrict__ out)
 {
     using L = CompLayout<M>;
-    constexpr int MM = M * M;
-    constexpr int E = packed_entries(M);
+    constexpr int EO = packed_entries(M, false), ET = packed_entries(M, true);
     extern __shared__ __attribute__((aligned(16))) unsigned char sm[];
     double *sq = reinterpret_cast<double *>(sm + L::sq);
     for (int x = threadIdx.x; x < b.N + 3; x += blockDim.x) sq[x] = sqrt((double)x);
@@ -251,7 +211,7 @@ __global__ __launch_bounds__(CR) __attribute__((amdgpu_waves_per_eu(2))) void k_
             pbl[J * CW + wave] = pb;
         }
         if (act) rows[(J * CW + wave) * 64 + __popcll(m & below)] = (unsigned char)lane;
-        pb += __popcll(m) * E;
+        pb += __popcll(m) * EO;
     }
     pb = 0;
     {
@@ -266,7 +226,7 @@ __global__ __launch_bounds__(CR) __attribute__((amdgpu_waves_per_eu(2))) void k_
                     pbl[J * CW + wave] = pb;
                 }
                 if (act) rows[(J * CW + wave) * 64 + __popcll(m & below)] = (unsigned char)lane;
-                pb += __popcll(m) * E;
+                pb += __popcll(m) * ET;
             }
         }
     }
@@ -275,12 +235,12 @@ __global__ __launch_bounds__(CR) __attribute__((amdgpu_waves_per_eu(2))) void k_
     // jump phase: same-orbital (rules 3, 6), then two-orbital (rules 7, 9)
     int bctr = 0;
     for (int k = 0; k < M; k++)
-        comp_jump<M, RC, false>(b.packOT + b0OT, C, (coef_ptr)(c.ot + (int64_t)k * MM), k, k, k, sm, sq, wave, lane, bctr);
+        comp_jump<M, RC, false>(b.packOT + b0OT, C, (coef_ptr)(c.otc + (int64_t)k * EO), k, k, k, sm, sq, wave, lane, bctr);
     {
         int J = M, p = 0;
         for (int k = 0; k < M; k++)
             for (int s = k + 1; s < M; s++, J++, p++)
-                comp_jump<M, RC, true>(b.packTT + b0TT, C, (coef_ptr)(c.tt + (int64_t)p * MM), J, k, s, sm, sq, wave, lane,
+                comp_jump<M, RC, true>(b.packTT + b0TT, C, (coef_ptr)(c.ttc + (int64_t)p * ET), J, k, s, sm, sq, wave, lane,
                                        bctr);
     }
     __syncthreads();

@@ -223,50 +223,96 @@ __device__ inline cplx chunk_ranked(const uint32_t (&D0)[M], const uint32_t (&D1
     return part;
 }
 
-// The same sum with the targets read from a packed slot: group g (entries
-// t = 4g .. 4g+3, t = l(l+1)/2 + q) of this lane at slot + g * nact4.  All
-// groups are loaded first and a scheduling barrier keeps the compiler from
-// sinking each group load next to its gathers: left to itself it issued a
-// group load, waited for it (vmcnt 0), issued its gathers, and so on, with a
-// few loads in flight; with all index loads up front the gathers of a chunk
-// pipeline 12-18 deep (N = M = 12: 5.95 ms -> 4.47 ms per H.v).
-template <int M, bool RC>
-__device__ inline cplx chunk_packed(const int32_t *__restrict__ slot, int nact4, const cplx *__restrict__ C,
-                                    const RowState<M> &r, const double *sq, const cplx *coef)
+// Occupations of a row with the removed orbitals k and s (s == k: one) taken
+// out and the rest renumbered o' = o - [k < o] - [s < o], 8 bits each: the
+// orbital numbering of a packed slot (packed_pos in mcpi_internal.h)
+__device__ inline void compress_occ(const uint64_t (&pk)[2], int k, int s, uint64_t (&out)[2])
 {
-    constexpr int G = packed_entries(M) / 4;
-    int32_t idx[4 * G];
+    typedef unsigned __int128 u128;
+    u128 x = ((u128)pk[1] << 64) | (u128)pk[0];
+    auto drop = [](u128 v, int o) {
+        const u128 lo = ((u128)1 << (8 * o)) - 1;
+        return (v & lo) | ((v >> 8) & ~lo);
+    };
+    if (s != k) x = drop(x, s);   // s > k: dropping s first leaves k in place
+    x = drop(x, k);
+    out[0] = (uint64_t)x;
+    out[1] = (uint64_t)(x >> 64);
+}
+
+template <int O>
+__device__ inline int occ_c(const uint64_t (&pkc)[2])
+{
+    return (int)((O < 8 ? (pkc[0] >> (8 * O)) : (pkc[1] >> (8 * (O - 8)))) & 0xff);
+}
+
+__device__ inline cplx ldcoef(const cplx *p, int x) { return p[x]; }
+__device__ inline cplx ldcoef(const __attribute__((address_space(4))) cplx *p, int x)
+{
+    return cmake(p[x].re, p[x].im);   // field by field (no generic-address copy)
+}
+
+// rows l' = L .. MP-1 of a packed chunk (compile-time recursion keeps every
+// index a constant)
+template <int M, bool RC, class CP, int MP, int L>
+__device__ inline void chunk_rows(const int32_t *idx, const cplx *__restrict__ C, const uint64_t (&pkc)[2],
+                                  const double *sq, CP coef, cplx &part)
+{
+    if constexpr (L < MP) {
+        cplx pl = cmake(0, 0);
+#pragma unroll
+        for (int q = 0; q <= L; q++) {
+            const int t = L * (L + 1) / 2 + q;
+            const cplx cj = ld_c(C + (int64_t)idx[t]);
+            const int vq = (int)((q < 8 ? (pkc[0] >> (8 * q)) : (pkc[1] >> (8 * (q - 8)))) & 0xff);
+            if (q < L) pl = term<RC>(ldcoef(coef, t), sq[vq + 1], cj, pl);
+            else part = term<RC>(ldcoef(coef, t), sq[vq + 1] * sq[vq + 2], cj, part);
+        }
+        const double rl = sq[occ_c<L>(pkc) + 1];
+        part.re = fma(rl, pl.re, part.re);
+        part.im = fma(rl, pl.im, part.im);
+        chunk_rows<M, RC, CP, MP, L + 1>(idx, C, pkc, sq, coef, part);
+    }
+}
+
+// The chunk sum of one (row, jump) pair from its packed slot: entry t of the
+// lane at tab + off + (t / 4) * nact4 + t % 4 (16-byte groups, off and the
+// stride 32-bit lane values from a uniform base), over the triangle of the
+// MP orbitals left when the jump's removed orbitals are dropped.  pkc: the
+// row's occupations in that numbering (compress_occ); coef: the jump's
+// coefficients in slot order (HamCoefDev::otc / ttc).  Row l':
+// sum_{q'<l'} coef r1[q'] C[j] is scaled once by r1[l']; q' == l' uses
+// sqrt((v+1)(v+2)).  Square roots come from the LDS table sq.
+// All groups are loaded first and a scheduling barrier keeps the compiler
+// from sinking each group load next to its gathers: left to itself it issued
+// a group load, waited for it (vmcnt 0), issued its gathers, and so on, with
+// a few loads in flight; with all index loads up front the gathers of a
+// chunk pipeline 12-18 deep (N = M = 12: 5.95 ms -> 4.47 ms per H.v).
+template <int M, bool RC, bool TT, class CP>
+__device__ inline cplx chunk_packed(const int32_t *__restrict__ tab, uint32_t off, uint32_t nact4,
+                                    const cplx *__restrict__ C, const uint64_t (&pkc)[2], const double *sq, CP coef)
+{
+    constexpr int MP = packed_orbitals(M, TT), G = packed_entries(M, TT) / 4;
+    int32_t idx[4 * G + 4];
 #pragma unroll
     for (int g = 0; g < G; g++) {
-        const Piece4 pc = ld_group_nt(slot + (int64_t)g * nact4);
+        const Piece4 pc = ld_group_nt(tab + (off + (uint32_t)g * nact4));
 #pragma unroll
         for (int u = 0; u < 4; u++) idx[4 * g + u] = pc.x[u];
     }
     __builtin_amdgcn_sched_barrier(0);
     cplx part = cmake(0, 0);
-#pragma unroll
-    for (int l = 0; l < M; l++) {
-        cplx pl = cmake(0, 0);
-#pragma unroll
-        for (int q = 0; q <= l; q++) {
-            const int t = l * (l + 1) / 2 + q;
-            const cplx cj = ld_c(C + (int64_t)idx[t]);
-            if (q < l) pl = term<RC>(coef[q + l * M], r.r1[q], cj, pl);
-            else part = term<RC>(coef[q + l * M], r.r1[q] * sq[occ<M>(r, q) + 2], cj, part);
-        }
-        part.re = fma(r.r1[l], pl.re, part.re);
-        part.im = fma(r.r1[l], pl.im, part.im);
-    }
+    chunk_rows<M, RC, CP, MP, 0>(idx, C, pkc, sq, coef, part);
     return part;
 }
-
 // the non-zero entries of one chunk from a uniform list (four at a time);
 // q and l are runtime values here, so occupations come from the packed copy,
 // square roots from the LDS table and prefix differences from the LDS copy
+// (DMAP: the slot of jump (k, s), entry (q, l) at packed_pos(k, s, q, l))
 template <int M, bool DMAP, bool RC>
 __device__ inline cplx chunk_list(const int32_t *__restrict__ slot, int nact4, const uint32_t *Dl, int lane, int T,
                                   uint32_t base, const cplx *__restrict__ C, const RowState<M> &r,
-                                  const double *sq, const int32_t *ents, const cplx *coefs, int n)
+                                  const double *sq, const int32_t *ents, const cplx *coefs, int n, int k, int s)
 {
     cplx part = cmake(0, 0);
     for (int t = 0; t < n; t += 4) {
@@ -279,7 +325,8 @@ __device__ inline cplx chunk_list(const int32_t *__restrict__ slot, int nact4, c
             const int e = ok ? ents[t + u] : 0;
             const int q = e % M, l = e / M;
             if (DMAP) {
-                const int t = l * (l + 1) / 2 + q;   // q <= l for every listed entry
+                // listed entries are rule terms: q <= l, neither in {k, s}
+                const int t = ok ? packed_pos(k, s, q, l) : 0;
                 j[u] = ld_nt(slot + (int64_t)(t >> 2) * nact4 + (t & 3));
             }
             else j[u] = (int64_t)(uint32_t)(base - dl<M>(Dl, lane, T, 0, q) - dl<M>(Dl, lane, T, 1, l));
@@ -436,7 +483,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(SKIP ? 4 : 
 
     // double jumps: slots of the packed tables (DMAP) follow the jumps
     // active in any lane, in the order visited below
-    constexpr int E = packed_entries(M);
+    constexpr int EO = packed_entries(M, false), ET = packed_entries(M, true);
     const int lw = threadIdx.x & 63;
     const uint64_t below = lw ? (~0ull >> (64 - lw)) : 0ull;
     const int64_t wv = i >> 6;
@@ -489,8 +536,10 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(SKIP ? 4 : 
             const bool act = vk >= 2;
             const uint64_t m = DMAP ? __ballot(act) : 0ull;
             const int nact = __popcll(m);
-            const int32_t *slot = DMAP ? b.packOT + pb + 4 * __popcll(m & below) : nullptr;
-            pb += (int64_t)nact * E;
+            const int32_t *slot0 = DMAP ? b.packOT + pb : nullptr;   // the jump's slot, uniform
+            const uint32_t soff = 4u * (uint32_t)__popcll(m & below);
+            const int32_t *slot = DMAP ? slot0 + soff : nullptr;
+            pb += (int64_t)nact * EO;
             if (SKIP && n_nz == 0) continue;
             if (!act) continue;
             uint32_t D0[M], D1[M], tot = 0;
@@ -500,11 +549,16 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(SKIP ? 4 : 
             const double f = sq[vk] * sq[vk - 1];
             cplx part;
             if (!SKIP) {
-                if (DMAP) part = chunk_packed<M, RC>(slot, 4 * nact, C, r, sq, c.ot + (int64_t)k * MM);
+                if (DMAP) {
+                    uint64_t pkc[2];
+                    compress_occ(r.pk, k, k, pkc);
+                    part = chunk_packed<M, RC, false>(slot0, soff, 4u * (uint32_t)nact, C, pkc, sq,
+                                                      (const cplx *)(c.otc + (int64_t)k * EO));
+                }
                 else part = chunk_ranked<M, RC>(D0, D1, base, C, r, sq, c.ot + (int64_t)k * MM);
             } else {
                 part = chunk_list<M, DMAP, RC>(slot, 4 * nact, Dl, lane, T, base, C, r, sq, c.nz_e + c.ot_off[k],
-                                               c.nz_c + c.ot_off[k], n_nz);
+                                               c.nz_c + c.ot_off[k], n_nz, k, k);
             }
             acc.re = fma(f, part.re, acc.re);
             acc.im = fma(f, part.im, acc.im);
@@ -522,8 +576,10 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(SKIP ? 4 : 
                 const bool act = vk >= 1 && vs >= 1;
                 const uint64_t m = DMAP ? __ballot(act) : 0ull;
                 const int nact = __popcll(m);
-                const int32_t *slot = DMAP ? b.packTT + pb + 4 * __popcll(m & below) : nullptr;
-                pb += (int64_t)nact * E;
+                const int32_t *slot0 = DMAP ? b.packTT + pb : nullptr;
+                const uint32_t soff = 4u * (uint32_t)__popcll(m & below);
+                const int32_t *slot = DMAP ? slot0 + soff : nullptr;
+                pb += (int64_t)nact * ET;
                 if (SKIP && n_nz == 0) continue;
                 if (!act) continue;
                 uint32_t D0[M], D1[M], tot = 0;
@@ -533,11 +589,16 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(SKIP ? 4 : 
                 const double f = sq[vk] * sq[vs];
                 cplx part;
                 if (!SKIP) {
-                    if (DMAP) part = chunk_packed<M, RC>(slot, 4 * nact, C, r, sq, c.tt + (int64_t)p * MM);
+                    if (DMAP) {
+                        uint64_t pkc[2];
+                        compress_occ(r.pk, k, s, pkc);
+                        part = chunk_packed<M, RC, true>(slot0, soff, 4u * (uint32_t)nact, C, pkc, sq,
+                                                         (const cplx *)(c.ttc + (int64_t)p * ET));
+                    }
                     else part = chunk_ranked<M, RC>(D0, D1, base, C, r, sq, c.tt + (int64_t)p * MM);
                 } else {
                     part = chunk_list<M, DMAP, RC>(slot, 4 * nact, Dl, lane, T, base, C, r, sq,
-                                                   c.nz_e + c.tt_off[p], c.nz_c + c.tt_off[p], n_nz);
+                                                   c.nz_e + c.tt_off[p], c.nz_c + c.tt_off[p], n_nz, k, s);
                 }
                 acc.re = fma(f, part.re, acc.re);
                 acc.im = fma(f, part.im, acc.im);

@@ -25,13 +25,13 @@ __device__ inline const T *shifted(const T *p, size_t bytes)
     return (const T *)((const char *)p + bytes);
 }
 
-// the non-zero entries of one chunk for NV instances (lists shared; each
-// instance's coefficients at its own block offset)
+// the non-zero entries of one chunk of jump (k, s) for NV instances (lists
+// shared; each instance's coefficients at its own block offset)
 template <int M, int NV, bool DMAP, bool RC>
 __device__ inline void chunk_list_multi(const int32_t *__restrict__ slot, int nact4, const uint32_t *Dl, int lane,
                                         int T, uint32_t base, const RowState<M> &r, const double *sq,
                                         const int32_t *ents, const cplx *coefs0, size_t cstride,
-                                        const cplx *__restrict__ C0, int64_t vstride, int nv, int n,
+                                        const cplx *__restrict__ C0, int64_t vstride, int nv, int n, int k, int s,
                                         cplx (&part)[NV])
 {
     for (int t = 0; t < n; t += 4) {
@@ -43,7 +43,7 @@ __device__ inline void chunk_list_multi(const int32_t *__restrict__ slot, int na
             const int e = ok ? ents[t + u] : 0;
             const int q = e % M, l = e / M;
             if (DMAP) {
-                const int tt = l * (l + 1) / 2 + q;
+                const int tt = ok ? packed_pos(k, s, q, l) : 0;   // listed: q, l not in {k, s}
                 j[u] = ld_nt(slot + (int64_t)(tt >> 2) * nact4 + (tt & 3));
             } else {
                 j[u] = (int64_t)(uint32_t)(base - dl<M>(Dl, lane, T, 0, q) - dl<M>(Dl, lane, T, 1, l));
@@ -183,7 +183,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void k
         }
     }
 
-    constexpr int E = packed_entries(M);
+    constexpr int EO = packed_entries(M, false), ET = packed_entries(M, true);
     const int lw = threadIdx.x & 63;
     const uint64_t below = lw ? (~0ull >> (64 - lw)) : 0ull;
     const int64_t wv = i >> 6;
@@ -251,7 +251,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void k
             const uint64_t m = DMAP ? __ballot(act) : 0ull;
             const int nact = __popcll(m);
             const int32_t *slot = DMAP ? b.packOT + pb + 4 * __popcll(m & below) : nullptr;
-            pb += (int64_t)nact * E;
+            pb += (int64_t)nact * EO;
             if (n_nz == 0 || !act) continue;
             uint32_t D0[M], D1[M], tot = 0;
             if (!DMAP) {
@@ -263,7 +263,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void k
 #pragma unroll
             for (int v = 0; v < NV; v++) part[v] = cmake(0, 0);
             chunk_list_multi<M, NV, DMAP, RC>(slot, 4 * nact, Dl, lane, T, top - tot, r, sq, c0.nz_e + c0.ot_off[k],
-                                              nzc + c0.ot_off[k], cstride, C, vstride, nv, n_nz, part);
+                                              nzc + c0.ot_off[k], cstride, C, vstride, nv, n_nz, k, k, part);
 #pragma unroll
             for (int v = 0; v < NV; v++) {
                 acc[v].re = fma(f, part[v].re, acc[v].re);
@@ -284,7 +284,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void k
                 const uint64_t m = DMAP ? __ballot(act) : 0ull;
                 const int nact = __popcll(m);
                 const int32_t *slot = DMAP ? b.packTT + pb + 4 * __popcll(m & below) : nullptr;
-                pb += (int64_t)nact * E;
+                pb += (int64_t)nact * ET;
                 if (n_nz == 0 || !act) continue;
                 uint32_t D0[M], D1[M], tot = 0;
                 if (!DMAP) {
@@ -297,7 +297,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void k
                 for (int v = 0; v < NV; v++) part[v] = cmake(0, 0);
                 chunk_list_multi<M, NV, DMAP, RC>(slot, 4 * nact, Dl, lane, T, top - tot, r, sq,
                                                   c0.nz_e + c0.tt_off[p], nzc + c0.tt_off[p], cstride, C, vstride,
-                                                  nv, n_nz, part);
+                                                  nv, n_nz, k, s, part);
 #pragma unroll
                 for (int v = 0; v < NV; v++) {
                     acc[v].re = fma(f, part[v].re, acc[v].re);

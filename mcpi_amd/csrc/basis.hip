@@ -289,12 +289,14 @@ __global__ void k_double_map(mcpi::BasisDev b)
 // lines.  The packed layout stores, for wavefront w (rows 64w .. 64w+63) and
 // each jump active in at least one lane (ballot != 0), one slot
 //     slot[g][a][u] = target of entry t = 4g + u for the a-th active lane
-// (t = l(l+1)/2 + q over q <= l, the half of the chunk H.v reads; entries
-// t >= M(M+1)/2 hold the row's own index).  A 16-byte load of all active
-// lanes is then one contiguous run.  Slots follow the reference's chunk
-// order (pairs k < s lexicographic; OT: k ascending), so a lane's slot
-// sequence is its row's chunk sequence with the rows of the wavefront
-// interleaved.
+// over the entries q <= l of the chunk (the half H.v reads) that are rule
+// terms: q and l outside the jump's removed orbitals {k, s}, at t = l'(l'+1)/2
+// + q' with the removed orbitals dropped from the numbering
+// (packed_pos in mcpi_internal.h); entries past the real ones hold the row's
+// own index.  A 16-byte load of all active lanes is then one contiguous
+// run.  Slots follow the reference's chunk order (pairs k < s
+// lexicographic; OT: k ascending), so a lane's slot sequence is its row's
+// chunk sequence with the rows of the wavefront interleaved.
 
 // ints per wavefront (counts fit int32: at most C(32,2) * 64 * 528)
 __global__ void k_packed_counts(int M, int64_t nc, const int32_t *__restrict__ IF, int32_t *__restrict__ cnt_ot,
@@ -317,8 +319,8 @@ __global__ void k_packed_counts(int M, int64_t nc, const int32_t *__restrict__ I
         for (int s = k + 1; s < M; s++) tt += __popcll(__ballot(valid && ((m1 >> k) & 1u) && ((m1 >> s) & 1u)));
     }
     if ((threadIdx.x & 63) == 0 && w < nw) {
-        cnt_ot[w] = ot * packed_entries(M);
-        cnt_tt[w] = tt * packed_entries(M);
+        cnt_ot[w] = ot * packed_entries(M, false);
+        cnt_tt[w] = tt * packed_entries(M, true);
     }
 }
 
@@ -347,7 +349,7 @@ __global__ void k_packed_fill(mcpi::BasisDev b)
         m1 |= (uint32_t)(x >= 1) << j;
         m2 |= (uint32_t)(x >= 2) << j;
     }
-    const int E = packed_entries(M), MH = M * (M + 1) / 2;
+    const int E = packed_entries(M, TT), MP = packed_orbitals(M, TT);
     int32_t *pack = TT ? b.packTT : b.packOT;
     int64_t base = (TT ? b.poffTT : b.poffOT)[w];
     for (int k = 0; k < M; k++) {
@@ -358,18 +360,14 @@ __global__ void k_packed_fill(mcpi::BasisDev b)
             const int nact = __popcll(m), my = __popcll(m & below);
             if (act) {
                 int t = 0;
-                for (int l = 0; l < M; l++)
-                    for (int q = 0; q <= l; q++, t++) {
-                        // an entry adding at a removed orbital is no double-jump
-                        // term (its folded coefficient is zero): the row's own
-                        // index keeps its gather coalesced and cached
-                        const bool excl = q == k || q == s || l == k || l == s;
+                for (int lp = 0; lp < MP; lp++)
+                    for (int qp = 0; qp <= lp; qp++, t++) {
+                        const int q = packed_orbital(k, s, qp), l = packed_orbital(k, s, lp);
                         pack[base + ((int64_t)(t >> 2) * nact + my) * 4 + (t & 3)] =
-                            excl ? (int32_t)i : (int32_t)rank_moved(P, Bt, N, M, b.nc, k, s, l, q);
+                            (int32_t)rank_moved(P, Bt, N, M, b.nc, k, s, l, q);
                     }
                 for (; t < E; t++) pack[base + ((int64_t)(t >> 2) * nact + my) * 4 + (t & 3)] = (int32_t)i;
             }
-            (void)MH;
             base += (int64_t)nact * E;
         }
     }

@@ -100,9 +100,8 @@ struct BasisDev {
     int32_t *packOT, *packTT;
     int64_t *poffOT, *poffTT;   // nw + 1 entries, nw = ceil(nc / 64)
     // Pattern tables of one zero pattern (set only in the copy a launch
-    // passes): the same wave-interleaved slots, holding only the targets of
-    // the listed (non-zero) terms of each jump, in list order, padded to a
-    // multiple of four
+    // passes): per wavefront and jump, entry-major slots holding only the
+    // targets of the listed (non-zero) terms of each jump, in list order
     int32_t *patOT, *patTT;
     int64_t *ppoffOT, *ppoffTT;
 };
@@ -117,9 +116,34 @@ struct PatternTables {
     uint64_t last_use = 0;
 };
 
-// entries of a packed slot: the M(M+1)/2 targets q <= l of a chunk, in
-// triangular order t = l(l+1)/2 + q, padded to a multiple of four
-__host__ __device__ constexpr int packed_entries(int M) { return (M * (M + 1) / 2 + 3) / 4 * 4; }
+// Entries of a packed slot.  A double jump removes particles at k and s
+// (s == k for a same-orbital jump) and adds them at q <= l; entries with q or
+// l in {k, s} are no rule term (their folded coefficients are zero), so a
+// slot holds only the others.  With the removed orbitals dropped, the
+// remaining MP = M - 1 (same-orbital) or M - 2 (two-orbital) orbitals are
+// renumbered o' = o - [k < o] - [s < o] (s counted once when s == k), and
+// entry (q, l) sits at t = l'(l'+1)/2 + q' of the triangle over MP
+// orbitals; the slot is padded to a multiple of four (padding holds the
+// row's own index).
+__host__ __device__ constexpr int packed_orbitals(int M, bool tt) { return tt ? (M >= 2 ? M - 2 : 0) : (M >= 1 ? M - 1 : 0); }
+__host__ __device__ constexpr int packed_real(int M, bool tt)
+{
+    return packed_orbitals(M, tt) * (packed_orbitals(M, tt) + 1) / 2;
+}
+__host__ __device__ constexpr int packed_entries(int M, bool tt) { return (packed_real(M, tt) + 3) / 4 * 4; }
+// position of entry (q, l), q <= l, neither in {k, s}, in a slot of jump (k, s)
+__host__ __device__ inline int packed_pos(int k, int s, int q, int l)
+{
+    const int qq = q - (k < q) - (s != k && s < q), ll = l - (k < l) - (s != k && s < l);
+    return ll * (ll + 1) / 2 + qq;
+}
+// orbital of renumbered index o' (inverse of the renumbering above)
+__host__ __device__ inline int packed_orbital(int k, int s, int op)
+{
+    int o = op + (op >= k);
+    if (s != k) o += (o >= s);
+    return o;
+}
 
 }  // namespace mcpi
 
@@ -150,6 +174,8 @@ struct HamCoefDev {
     cplx *A;     // M*M*M  ((k + l*M)*M + s)
     cplx *ot;    // M * M*M
     cplx *tt;    // P * M*M, P = M(M-1)/2 pairs in lexicographic order
+    cplx *otc;   // M * packed_entries(M, false): ot in packed-slot order
+    cplx *ttc;   // P * packed_entries(M, true): tt in packed-slot order
     uint8_t *nz_1j;  // M*M: 1 if any coefficient of the jump k->l is non-zero
     uint8_t *nz_ot;  // M*M*M: per (k, e)
     uint8_t *nz_tt;  // P*M*M: per (p, e)
@@ -178,6 +204,8 @@ __host__ __device__ inline HamCoefDev shift_coef(HamCoefDev c, ptrdiff_t d)
     c.A = sh(c.A);
     c.ot = sh(c.ot);
     c.tt = sh(c.tt);
+    c.otc = sh(c.otc);
+    c.ttc = sh(c.ttc);
     c.nz_1j = sh(c.nz_1j);
     c.nz_ot = sh(c.nz_ot);
     c.nz_tt = sh(c.nz_tt);

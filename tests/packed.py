@@ -1,15 +1,31 @@
 """Test-side restatement of the wave-packed double-jump layout (basis.hip,
 k_packed_counts / k_packed_fill): built from the reference-layout maps
 (configurationsMap.h:421-762 chunk order) so that the GPU tables can be
-compared entry by entry.  Entries whose added orbital q or l is one of the
-chunk's removed orbitals k, s are no double-jump term (the rules fold them
-into zero coefficients) and hold the row's own index instead of the map's
-target."""
+compared entry by entry.  A slot of jump (k, s) (s == k for a same-orbital
+jump) holds the entries q <= l of the chunk with neither q nor l in {k, s}
+(the others are no double-jump term: the rules fold them into zero
+coefficients), in triangular order over the orbitals left when k and s are
+dropped, padded to a multiple of four with the row's own index."""
 import numpy as np
 
 
-def entries(M: int) -> int:
-    return (M * (M + 1) // 2 + 3) // 4 * 4
+def kept_orbitals(M: int, k: int, s: int):
+    return [o for o in range(M) if o != k and o != s]
+
+
+def real_entries(M: int, tt: bool) -> int:
+    mp = max(M - (2 if tt else 1), 0)
+    return mp * (mp + 1) // 2
+
+
+def entries(M: int, tt: bool) -> int:
+    return (real_entries(M, tt) + 3) // 4 * 4
+
+
+def slot_pairs(M: int, k: int, s: int):
+    """(q, l) of each real slot entry, in slot order."""
+    kept = kept_orbitals(M, k, s)
+    return [(kept[qp], kept[lp]) for lp in range(len(kept)) for qp in range(lp + 1)]
 
 
 def jumps(M: int, tt: bool):
@@ -22,8 +38,7 @@ def expected_wave(IF, M, rows, Map, stride, stride_base, tt: bool):
     """Packed ints of one wavefront (rows: consecutive row indices, at most
     64).  Map/stride: reference-layout chunk map and its per-row offsets
     (stride[r - stride_base] is row r's first entry in Map)."""
-    E = entries(M)
-    tri = [(l, q) for l in range(M) for q in range(l + 1)]
+    E = entries(M, tt)
     V = IF[rows]
     ordinal = np.zeros(len(rows), np.int64)
     out = []
@@ -32,12 +47,12 @@ def expected_wave(IF, M, rows, Map, stride, stride_base, tt: bool):
         if not act.any():
             continue
         idx = np.nonzero(act)[0]
+        pairs = slot_pairs(M, k, s)
         slot = np.zeros((E // 4, len(idx), 4), np.int64)
         for a, ri in enumerate(idx):
             r = rows[ri]
             beg = stride[r - stride_base] + ordinal[ri] * M * M
-            # entries adding at a removed orbital (q or l in {k, s}) hold r
-            vals = [r if {q, l} & {k, s} else Map[beg + l + q * M] for (l, q) in tri] + [r] * (E - len(tri))
+            vals = [Map[beg + l + q * M] for (q, l) in pairs] + [r] * (E - len(pairs))
             slot[:, a, :] = np.asarray(vals).reshape(E // 4, 4)
             ordinal[ri] += 1
         out.append(slot.reshape(-1))

@@ -145,7 +145,7 @@ def test_full_maps_n12_m12(oracle_lib):
         poff = g.copy_range("poff" + name, 0, nw + 1)
         assert poff[0] == 0 and poff[-1] == (g.ptt_len if tt else g.pot_len)
         assert np.all(np.diff(poff) >= 0)
-        assert poff[-1] == n_dbl * packed.entries(M)   # one slot row per reference chunk
+        assert poff[-1] == n_dbl * packed.entries(M, tt)   # one slot row per reference chunk
         for w in np.concatenate([[0, nw - 1], rng.integers(0, nw, 12)]):
             r0, r1 = 64 * int(w), min(64 * int(w) + 64, g.nc)
             rows_w = list(range(r0, r1))

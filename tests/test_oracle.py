@@ -337,15 +337,39 @@ def test_bethe_solver(oracle_lib):
 @pytest.mark.parametrize("N,M", [(4, 5), (6, 4), (3, 7)])
 def test_packed_layout_restatement(oracle_lib, N, M):
     """The test-side wave-packed layout holds every reference chunk once:
-    E entries per chunk, offsets increasing, and each slot row decodes to the
-    q <= l half of its chunk."""
+    E entries per chunk, offsets increasing; a slot keeps the entries q <= l
+    outside the removed orbitals, and each one, decoded by position, is the
+    rank of v - e_k - e_s + e_q + e_l (brute force through the oracle)."""
     from tests import packed
     o = oracle.Basis(N, M)
     IF = o.IF.reshape(-1, M)
+    T = oracle.ncmat(N, M)
     for tt, Mp, st in ((False, o.MapOT, o.strideOT), (True, o.MapTT, o.strideTT)):
         pack, poff = packed.expected_packed(IF, M, Mp, st, tt)
-        assert pack.size == Mp.size // (M * M) * packed.entries(M)
+        assert pack.size == Mp.size // (M * M) * packed.entries(M, tt)
         assert poff[0] == 0 and poff[-1] == pack.size and np.all(np.diff(poff) >= 0)
-        # row 0's first chunk (if any) sits in wave 0's first slot, lane 0
-        if st.size and (st[1] if st.size > 1 else Mp.size) > st[0]:
-            assert pack[0] == Mp[0]
+        # wave 0: walk its slots in jump order and check every entry of every
+        # active lane against the rank of the moved state
+        rows = list(range(min(64, IF.shape[0])))
+        E, off = packed.entries(M, tt), 0
+        for (k, s) in packed.jumps(M, tt):
+            act = [r for r in rows if (IF[r, k] >= 1 and IF[r, s] >= 1 if tt else IF[r, k] >= 2)]
+            if not act:
+                continue
+            pairs = packed.slot_pairs(M, k, s)
+            assert len(pairs) == packed.real_entries(M, tt)
+            for a, r in enumerate(act):
+                for t in range(E):
+                    got = pack[off + ((t // 4) * len(act) + a) * 4 + t % 4]
+                    if t < len(pairs):
+                        q, l = pairs[t]
+                        v = IF[r].copy()
+                        v[k] -= 1
+                        v[s] -= 1
+                        v[q] += 1
+                        v[l] += 1
+                        assert got == oracle.rank(N, M, T, v), (k, s, q, l, r)
+                    else:
+                        assert got == r
+            off += E * len(act)
+        assert off == poff[1]
