@@ -34,9 +34,10 @@ namespace {
 #define MCPI_CONST __attribute__((address_space(4)))
 typedef const MCPI_CONST cplx *coef_ptr;
 
-// wavefronts per block: 4 (43.6 KB of LDS at M = 12, two blocks per CU);
-// 8 needs 119 KB, one block per CU, and was slower (3.81 ms instead of 3.37
-// at N = M = 12) although its batches are 9% fewer
+// wavefronts per block: 4 (43.6 KB of LDS at M = 12, three blocks per CU).
+// Larger blocks need fewer batches (6% fewer with 6 wavefronts, 9% with 8)
+// but were slower at N = M = 12: 2.83 ms instead of 2.43 with 6 (77 KB, two
+// blocks per CU), 3.81 instead of 3.37 with 8 (119 KB, one block per CU)
 constexpr int CW = 4;
 constexpr int CR = 64 * CW;    // rows per block
 
