@@ -6,10 +6,11 @@
 // The reference loops orbital-outer / row-inner.  Here a block walks row
 // tiles (lanes = rows) and, for every representative term of a class, sums
 // the lanes' contributions with a wavefront reduction and adds the result to
-// an LDS accumulator.  Each block writes its accumulators once; a second
-// kernel sums the blocks in a fixed order, so the result is deterministic.
-// The host expands the representative terms into full matrices with the
-// reference's symmetry copies.
+// an LDS accumulator (atomically: the wavefronts of a block add in the order
+// they arrive, so the last bits of a sum can differ from run to run).  Each
+// block writes its accumulators once; a second kernel sums the blocks in a
+// fixed order.  The host expands the representative terms into full
+// matrices with the reference's symmetry copies.
 //
 // Targets come from the stored maps when the basis has them and are ranked
 // on the fly otherwise (closed-form FockToIndex over the lane's prefix
@@ -71,128 +72,246 @@ __device__ inline int64_t rank_lane(const uint8_t *Vs, int t, int T, const int32
     return nc - 1 - s;
 }
 
-template <int CLASS>
-__global__ __launch_bounds__(256) void k_rho(BasisDev b, const cplx *__restrict__ C, int p0, int p1,
-                                            int nterms, cplx *__restrict__ partial)
+// U terms (q[u], l[u]) of one double jump (k, s), rows r < R of the lane:
+// every target first (map entries or ranks), then every gather, then the
+// sums and U wavefront reductions, so that the memory accesses of the batch
+// are in flight together instead of one dependent chain per term.
+// tab: MapOT or MapTT (nullptr: rank on the fly); base[r]: the row's chunk
+// offset in it; f[r]: the jump's factor (0 for rows without the jump);
+// diag: q == l terms (sqrt((v+1)(v+2)) instead of sqrt(v_q+1) sqrt(v_l+1)).
+template <int U, int R>
+__device__ inline void rho_batch(const int32_t *tab, const int64_t (&base)[R], const bool (&act)[R],
+                                 const double (&f)[R], const cplx (&ci)[R], const cplx *__restrict__ C,
+                                 const uint8_t *Vs, int t, int T, int TR, const int32_t *Bt, int N, int M, int64_t nc,
+                                 const double *sq, int k, int s, const int (&q)[U], const int (&l)[U], int nu,
+                                 bool diag, cplx *acc, const int (&slot)[U])
+{
+    int64_t j[U][R];
+#pragma unroll
+    for (int u = 0; u < U; u++)
+#pragma unroll
+        for (int r = 0; r < R; r++) {
+            const int qq = u < nu ? q[u] : q[0], ll = u < nu ? l[u] : l[0];
+            j[u][r] = 0;
+            if (act[r])
+                j[u][r] = tab ? (int64_t)tab[base[r] + ll + qq * M]
+                              : rank_lane(Vs, t + T * r, TR, Bt, N, M, nc, k, s, qq, ll);
+        }
+    cplx g[U][R];
+#pragma unroll
+    for (int u = 0; u < U; u++)
+#pragma unroll
+        for (int r = 0; r < R; r++) g[u][r] = act[r] ? ldcr(C + j[u][r]) : cmake(0, 0);
+#pragma unroll
+    for (int u = 0; u < U; u++) {
+        if (u >= nu) break;   // uniform
+        cplx z = cmake(0, 0);
+#pragma unroll
+        for (int r = 0; r < R; r++) {
+            const int vq = Vs[q[u] * TR + t + T * r], vl = Vs[l[u] * TR + t + T * r];
+            const double x = diag ? sq[vq + 1] * sq[vq + 2] : sq[vq + 1] * sq[vl + 1];
+            z = cfma(cfma_conj(ci[r], g[u][r], cmake(0, 0)), cmake(f[r] * x, 0), z);
+        }
+        wave_add(z, acc, slot[u], true);
+    }
+}
+
+// A block of T = 1024 threads (16 wavefronts: the accumulators, up to
+// 120 KB of LDS, allow one block per CU, so the block has to fill the CU)
+// walks tiles of T*R rows; lane t holds rows tile + t + T*r, r < R, and sums
+// its R rows' contributions to a term before the wavefront reduction, which
+// divides the number of reductions and LDS atomics by R.
+#define RHO_U 8   // terms per batch in classes 1 and 2
+
+template <int CLASS, int R>
+__global__ __launch_bounds__(1024) void k_rho(BasisDev b, const cplx *__restrict__ C, int p0, int p1,
+                                             int nterms, cplx *__restrict__ partial)
 {
     extern __shared__ double smem_r[];
-    const int M = b.M, N = b.N, T = blockDim.x, t = threadIdx.x;
+    const int M = b.M, N = b.N, T = blockDim.x, t = threadIdx.x, TR = T * R;
     const int P = M * (M - 1) / 2;
     const int64_t nc = b.nc, MM = (int64_t)M * M;
     cplx *acc = (cplx *)smem_r;                       // nterms
     double *sq = (double *)(acc + nterms);            // N+3
     int32_t *Bt = (int32_t *)(sq + N + 3);            // M*(N+1) rank table
-    uint8_t *Vs = (uint8_t *)(Bt + M * (N + 1));      // M*T
+    uint8_t *Vs = (uint8_t *)(Bt + M * (N + 1));      // M*T*R occupations, [j][row of tile]
     for (int x = t; x < nterms; x += T) acc[x] = cmake(0, 0);
     for (int x = t; x < N + 3; x += T) sq[x] = sqrt((double)x);
     for (int x = t; x < M * (N + 1); x += T) Bt[x] = b.B[x];
     __syncthreads();
 
-    for (int64_t tile = (int64_t)blockIdx.x * T; tile < nc; tile += (int64_t)gridDim.x * T) {
-        const int64_t i = tile + t;
-        const bool valid = i < nc;
-        for (int j = 0; j < M; j++) Vs[j * T + t] = (uint8_t)(valid ? b.IF[i * M + j] : 0);
+    for (int64_t tile = (int64_t)blockIdx.x * TR; tile < nc; tile += (int64_t)gridDim.x * TR) {
+        int64_t i[R];
+        bool valid[R];
+        cplx ci[R];
+        double m2[R];
+#pragma unroll
+        for (int r = 0; r < R; r++) {
+            i[r] = tile + t + (int64_t)T * r;
+            valid[r] = i[r] < nc;
+            for (int j = 0; j < M; j++) Vs[j * TR + t + T * r] = (uint8_t)(valid[r] ? b.IF[i[r] * M + j] : 0);
+            ci[r] = valid[r] ? ldcr(C + i[r]) : cmake(0, 0);
+            m2[r] = ci[r].re * ci[r].re + ci[r].im * ci[r].im;
+        }
         __syncthreads();
-#define V(j) ((int)Vs[(j) * T + t])
-        const cplx ci = valid ? ldcr(C + i) : cmake(0, 0);
-        const cplx cci = cmake(ci.re, -ci.im);
-        const double m2 = ci.re * ci.re + ci.im * ci.im;
+#define V(j, r) ((int)Vs[(j) * TR + t + T * (r)])
         if (CLASS == 0) {
             int o = 0;
             // r1d, R1
             for (int k = 0; k < M; k++) {
-                int vk = V(k);
-                wave_add(cmake(m2 * vk, 0), acc, o + k, true);
-                wave_add(cmake(m2 * vk * (vk - 1), 0), acc, o + P + M + k, wave_any(vk >= 2));
+                double a = 0, a2 = 0;
+                bool any2 = false;
+#pragma unroll
+                for (int r = 0; r < R; r++) {
+                    const int vk = V(k, r);
+                    a += m2[r] * vk;
+                    a2 += m2[r] * vk * (vk - 1);
+                    any2 = any2 || vk >= 2;
+                }
+                wave_add(cmake(a, 0), acc, o + k, true);
+                wave_add(cmake(a2, 0), acc, o + P + M + k, wave_any(any2));
             }
             o += M;  // r1o at o
             const int o_r1o = o, o_R2 = M + P + M, o_R4 = o_R2 + P, o_R5 = o_R4 + P, o_R7 = o_R5 + P;
             int pq = 0;
             for (int k = 0; k < M; k++)
                 for (int l = k + 1; l < M; l++, pq++) {
-                    const int vk = V(k), vl = V(l);
                     // R2: |C|^2 v_k v_l over all rows
-                    wave_add(cmake(m2 * vk * vl, 0), acc, o_R2 + pq, wave_any(vk && vl));
-                    const bool act = valid && vk >= 1;
-                    if (!wave_any(act)) continue;
-                    cplx g = cmake(0, 0);
-                    if (act) {
-                        const int64_t j = b.Map ? (int64_t)b.Map[i + (int64_t)(k + M * l) * nc]
-                                                : rank_lane(Vs, t, T, Bt, N, M, nc, k, -1, l, -1);
-                        g = cfma_conj(ci, ldcr(C + j), cmake(0, 0));   // conj(C_i) C_j
+                    double a = 0;
+                    bool any = false, anyact = false;
+#pragma unroll
+                    for (int r = 0; r < R; r++) {
+                        const int vk = V(k, r), vl = V(l, r);
+                        a += m2[r] * vk * vl;
+                        any = any || (vk && vl);
+                        anyact = anyact || (valid[r] && vk >= 1);
                     }
-                    const double root = act ? sq[vk] * sq[vl + 1] : 0.0;
-                    wave_add(cscale(g, root), acc, o_r1o + pq, true);
-                    wave_add(cscale(g, act && vk >= 2 ? (vk - 1) * root : 0.0), acc, o_R4 + pq,
-                             wave_any(act && vk >= 2));
-                    wave_add(cscale(g, act && vl >= 1 ? vl * root : 0.0), acc, o_R5 + pq,
-                             wave_any(act && vl >= 1));
+                    wave_add(cmake(a, 0), acc, o_R2 + pq, wave_any(any));
+                    if (!wave_any(anyact)) continue;
+                    cplx g[R];
+                    double root[R];
+                    cplx z1 = cmake(0, 0), z4 = cmake(0, 0), z5 = cmake(0, 0);
+                    bool any4 = false, any5 = false;
+#pragma unroll
+                    for (int r = 0; r < R; r++) {
+                        const int vk = V(k, r), vl = V(l, r);
+                        const bool act = valid[r] && vk >= 1;
+                        g[r] = cmake(0, 0);
+                        if (act) {
+                            const int64_t j = b.Map ? (int64_t)b.Map[i[r] + (int64_t)(k + M * l) * nc]
+                                                    : rank_lane(Vs, t + T * r, TR, Bt, N, M, nc, k, -1, l, -1);
+                            g[r] = cfma_conj(ci[r], ldcr(C + j), cmake(0, 0));   // conj(C_i) C_j
+                        }
+                        root[r] = act ? sq[vk] * sq[vl + 1] : 0.0;
+                        z1 = cfma(g[r], cmake(root[r], 0), z1);
+                        z4 = cfma(g[r], cmake(act && vk >= 2 ? (vk - 1) * root[r] : 0.0, 0), z4);
+                        z5 = cfma(g[r], cmake(act && vl >= 1 ? vl * root[r] : 0.0, 0), z5);
+                        any4 = any4 || (act && vk >= 2);
+                        any5 = any5 || (act && vl >= 1);
+                    }
+                    wave_add(z1, acc, o_r1o + pq, true);
+                    wave_add(z4, acc, o_R4 + pq, wave_any(any4));
+                    wave_add(z5, acc, o_R5 + pq, wave_any(any5));
                     for (int s = 0; s < M; s++) {
                         if (s == k || s == l) continue;
-                        const int vs = V(s);
-                        wave_add(cscale(g, act && vs >= 1 ? vs * root : 0.0), acc, o_R7 + pq * M + s,
-                                 wave_any(act && vs >= 1));
+                        cplx z7 = cmake(0, 0);
+                        bool any7 = false;
+#pragma unroll
+                        for (int r = 0; r < R; r++) {
+                            const int vs = V(s, r);
+                            const bool act = valid[r] && V(k, r) >= 1;
+                            z7 = cfma(g[r], cmake(act && vs >= 1 ? vs * root[r] : 0.0, 0), z7);
+                            any7 = any7 || (act && vs >= 1);
+                        }
+                        wave_add(z7, acc, o_R7 + pq * M + s, wave_any(any7));
                     }
                 }
         } else if (CLASS == 1) {
             // R3 (k < q) and R6 (q < l, k not in {q, l}), v_k >= 2
-            int cot = 0;
+            int cot[R];
+#pragma unroll
+            for (int r = 0; r < R; r++) cot[r] = 0;
             for (int k = 0; k < M; k++) {
-                const int vk = V(k);
-                const bool act = valid && vk >= 2;
-                const int64_t base = (act && b.MapOT) ? b.strideOT[i] + (int64_t)cot * MM : 0;
-                if (act) cot++;
-                if (!wave_any(act)) continue;
-                const double fk = act ? sq[vk] * sq[vk - 1] : 0.0;
-                for (int q = k + 1; q < M; q++) {
-                    cplx z = cmake(0, 0);
-                    if (act) {
-                        const int64_t j = b.MapOT ? (int64_t)b.MapOT[base + q + q * M]
-                                                  : rank_lane(Vs, t, T, Bt, N, M, nc, k, k, q, q);
-                        z = cscale(cfma_conj(ci, ldcr(C + j), cmake(0, 0)),
-                                   fk * sq[V(q) + 1] * sq[V(q) + 2]);
-                    }
-                    wave_add(z, acc, pair_index(k, q, M), true);
+                bool act[R], anyact = false;
+                int64_t base[R];
+                double fk[R];
+#pragma unroll
+                for (int r = 0; r < R; r++) {
+                    const int vk = V(k, r);
+                    act[r] = valid[r] && vk >= 2;
+                    base[r] = (act[r] && b.MapOT) ? b.strideOT[i[r]] + (int64_t)cot[r] * MM : 0;
+                    if (act[r]) cot[r]++;
+                    fk[r] = act[r] ? sq[vk] * sq[vk - 1] : 0.0;
+                    anyact = anyact || act[r];
                 }
+                if (!wave_any(anyact)) continue;
+                constexpr int U = RHO_U;
+                int qs[U], ls[U], sl[U], nu = 0;
+                // R3: q == l (k < q); the map entry q + q M is read as l + q M with l = q
+                for (int q = k + 1; q < M; q++) {
+                    qs[nu] = q; ls[nu] = q; sl[nu] = pair_index(k, q, M);
+                    if (++nu == U) {
+                        rho_batch<U, R>(b.MapOT, base, act, fk, ci, C, Vs, t, T, TR, Bt, N, M, nc, sq, k, k, qs, ls,
+                                        nu, true, acc, sl);
+                        nu = 0;
+                    }
+                }
+                if (nu) rho_batch<U, R>(b.MapOT, base, act, fk, ci, C, Vs, t, T, TR, Bt, N, M, nc, sq, k, k, qs, ls, nu,
+                                        true, acc, sl);
+                nu = 0;
+                // R6: q < l, k not in {q, l}; map entry l + q M
                 int pq = 0;
                 for (int q = 0; q < M; q++)
                     for (int l = q + 1; l < M; l++, pq++) {
                         if (q == k || l == k) continue;
-                        cplx z = cmake(0, 0);
-                        if (act) {
-                            const int64_t j = b.MapOT ? (int64_t)b.MapOT[base + l + q * M]
-                                                      : rank_lane(Vs, t, T, Bt, N, M, nc, k, k, q, l);
-                            z = cscale(cfma_conj(ci, ldcr(C + j), cmake(0, 0)),
-                                       fk * sq[V(q) + 1] * sq[V(l) + 1]);
+                        qs[nu] = q; ls[nu] = l; sl[nu] = P + k * P + pq;
+                        if (++nu == U) {
+                            rho_batch<U, R>(b.MapOT, base, act, fk, ci, C, Vs, t, T, TR, Bt, N, M, nc, sq, k, k, qs,
+                                            ls, nu, false, acc, sl);
+                            nu = 0;
                         }
-                        wave_add(z, acc, P + k * P + pq, true);
                     }
+                if (nu) rho_batch<U, R>(b.MapOT, base, act, fk, ci, C, Vs, t, T, TR, Bt, N, M, nc, sq, k, k, qs, ls, nu,
+                                        false, acc, sl);
             }
         } else {
             // R8: k < s occupied, q < l, {q, l} disjoint from {k, s}
-            int ctt = 0, p = 0;
+            int ctt[R];
+#pragma unroll
+            for (int r = 0; r < R; r++) ctt[r] = 0;
+            int p = 0;
             for (int k = 0; k < M; k++)
                 for (int s = k + 1; s < M; s++, p++) {
-                    const int vk = V(k), vs = V(s);
-                    const bool act = valid && vk >= 1 && vs >= 1;
-                    const int64_t base = (act && b.MapTT) ? b.strideTT[i] + (int64_t)ctt * MM : 0;
-                    if (act) ctt++;
+                    bool act[R], anyact = false;
+                    int64_t base[R];
+                    double fks[R];
+#pragma unroll
+                    for (int r = 0; r < R; r++) {
+                        const int vk = V(k, r), vs = V(s, r);
+                        act[r] = valid[r] && vk >= 1 && vs >= 1;
+                        base[r] = (act[r] && b.MapTT) ? b.strideTT[i[r]] + (int64_t)ctt[r] * MM : 0;
+                        if (act[r]) ctt[r]++;
+                        fks[r] = act[r] ? sq[vk] * sq[vs] : 0.0;
+                        anyact = anyact || act[r];
+                    }
                     if (p < p0 || p >= p1) continue;
-                    if (!wave_any(act)) continue;
-                    const double fks = act ? sq[vk] * sq[vs] : 0.0;
+                    if (!wave_any(anyact)) continue;
+                    constexpr int U = RHO_U;
+                    int qs[U], ls[U], sl[U], nu = 0;
                     int pq = 0;
+                    // MapTT entry q + l M (the transposed entry l + q M holds the same target)
                     for (int q = 0; q < M; q++)
                         for (int l = q + 1; l < M; l++, pq++) {
                             if (q == k || q == s || l == k || l == s) continue;
-                            cplx z = cmake(0, 0);
-                            if (act) {
-                                const int64_t j = b.MapTT ? (int64_t)b.MapTT[base + q + l * M]
-                                                          : rank_lane(Vs, t, T, Bt, N, M, nc, k, s, q, l);
-                                z = cscale(cfma_conj(ci, ldcr(C + j), cmake(0, 0)),
-                                           fks * sq[V(q) + 1] * sq[V(l) + 1]);
+                            qs[nu] = l; ls[nu] = q; sl[nu] = (p - p0) * P + pq;
+                            if (++nu == U) {
+                                rho_batch<U, R>(b.MapTT, base, act, fks, ci, C, Vs, t, T, TR, Bt, N, M, nc, sq, k, s,
+                                                qs, ls, nu, false, acc, sl);
+                                nu = 0;
                             }
-                            wave_add(z, acc, (p - p0) * P + pq, true);
                         }
+                    if (nu) rho_batch<U, R>(b.MapTT, base, act, fks, ci, C, Vs, t, T, TR, Bt, N, M, nc, sq, k, s, qs, ls,
+                                            nu, false, acc, sl);
                 }
         }
 #undef V
@@ -215,32 +334,45 @@ __global__ void k_sum_blocks(int nblocks, int nterms, const cplx *__restrict__ p
 }
 
 // run one class and bring its compact terms to the host
+template <int CLASS, int R>
+static void launch_rho(const BasisDev &b, const cplx *C, int p0, int p1, int nterms, cplx *partial, int nblocks,
+                       int T, size_t sm)
+{
+    (void)hipFuncSetAttribute((const void *)k_rho<CLASS, R>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)sm);
+    hipLaunchKernelGGL((k_rho<CLASS, R>), dim3(nblocks), dim3(T), sm, 0, b, C, p0, p1, nterms, partial);
+}
+
 static mcpi_status run_class(mcpi_basis bs, int cls, const cplx *C, int p0, int p1, int nterms,
                              std::vector<cplx> &host)
 {
     const BasisDev &b = bs->d;
-    const int T = 256;
+    const int T = 1024;
     int dev = 0, ncu = 0;
     MCPI_HIP(hipGetDevice(&dev));
     MCPI_HIP(hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev));
-    int64_t tiles = (b.nc + T - 1) / T;
-    int nblocks = (int)std::min<int64_t>(tiles, 2 * (int64_t)ncu);
-    size_t sm = sizeof(cplx) * nterms + sizeof(double) * (b.N + 3) + sizeof(int32_t) * b.M * (b.N + 1) +
-                (size_t)b.M * T + 16;
-    size_t max_sm = 160 * 1024;
+    const size_t max_sm = 160 * 1024;
+    const size_t base_sm = sizeof(cplx) * nterms + sizeof(double) * (b.N + 3) + sizeof(int32_t) * b.M * (b.N + 1) + 16;
+    // rows per lane: as many as the occupation tile leaves room for, at most
+    // 4 for class 0 and 2 for the batched double-jump classes (4 spills)
+    int R = cls == 0 ? 4 : 2;
+    while (R > 1 && base_sm + (size_t)b.M * T * R > max_sm) R /= 2;
+    const size_t sm = base_sm + (size_t)b.M * T * R;
     MCPI_ARG(sm <= max_sm, "density-matrix accumulators exceed LDS");
+    int64_t tiles = (b.nc + (int64_t)T * R - 1) / ((int64_t)T * R);
+    int nblocks = (int)std::min<int64_t>(std::max<int64_t>(tiles, 1), (int64_t)ncu);
     cplx *partial = nullptr, *out = nullptr;
     MCPI_HIP(hipMalloc(&partial, sizeof(cplx) * (size_t)nblocks * nterms));
     MCPI_HIP(hipMalloc(&out, sizeof(cplx) * nterms));
     if (cls == 0) {
-        MCPI_HIP(hipFuncSetAttribute((const void *)k_rho<0>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)sm));
-        hipLaunchKernelGGL(k_rho<0>, dim3(nblocks), dim3(T), sm, 0, b, C, p0, p1, nterms, partial);
+        if (R == 4) launch_rho<0, 4>(b, C, p0, p1, nterms, partial, nblocks, T, sm);
+        else if (R == 2) launch_rho<0, 2>(b, C, p0, p1, nterms, partial, nblocks, T, sm);
+        else launch_rho<0, 1>(b, C, p0, p1, nterms, partial, nblocks, T, sm);
     } else if (cls == 1) {
-        MCPI_HIP(hipFuncSetAttribute((const void *)k_rho<1>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)sm));
-        hipLaunchKernelGGL(k_rho<1>, dim3(nblocks), dim3(T), sm, 0, b, C, p0, p1, nterms, partial);
+        if (R == 2) launch_rho<1, 2>(b, C, p0, p1, nterms, partial, nblocks, T, sm);
+        else launch_rho<1, 1>(b, C, p0, p1, nterms, partial, nblocks, T, sm);
     } else {
-        MCPI_HIP(hipFuncSetAttribute((const void *)k_rho<2>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)sm));
-        hipLaunchKernelGGL(k_rho<2>, dim3(nblocks), dim3(T), sm, 0, b, C, p0, p1, nterms, partial);
+        if (R == 2) launch_rho<2, 2>(b, C, p0, p1, nterms, partial, nblocks, T, sm);
+        else launch_rho<2, 1>(b, C, p0, p1, nterms, partial, nblocks, T, sm);
     }
     MCPI_HIP(hipGetLastError());
     hipLaunchKernelGGL(k_sum_blocks, dim3((nterms + 255) / 256), dim3(256), 0, 0, nblocks, nterms, partial, out);
