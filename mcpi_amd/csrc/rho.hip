@@ -76,11 +76,14 @@ __device__ inline int64_t rank_lane(const uint8_t *Vs, int t, int T, const int32
 // every target first (map entries or ranks), then every gather, then the
 // sums and U wavefront reductions, so that the memory accesses of the batch
 // are in flight together instead of one dependent chain per term.
-// tab: MapOT or MapTT (nullptr: rank on the fly); base[r]: the row's chunk
-// offset in it; f[r]: the jump's factor (0 for rows without the jump);
-// diag: q == l terms (sqrt((v+1)(v+2)) instead of sqrt(v_q+1) sqrt(v_l+1)).
+// Targets: packed (pk[r] != nullptr: the row's slot in the wave-packed
+// tables, entry packed_pos(k, s, q, l) at group stride n4[r]), else tab
+// (MapOT or MapTT at the row's chunk offset base[r]), else ranked on the
+// fly.  f[r]: the jump's factor (0 for rows without the jump); diag: q == l
+// terms (sqrt((v+1)(v+2)) instead of sqrt(v_q+1) sqrt(v_l+1)).
 template <int U, int R>
-__device__ inline void rho_batch(const int32_t *tab, const int64_t (&base)[R], const bool (&act)[R],
+__device__ inline void rho_batch(const int32_t *const (&pk)[R], const uint32_t (&n4)[R], const int32_t *tab,
+                                 const int64_t (&base)[R], const bool (&act)[R],
                                  const double (&f)[R], const cplx (&ci)[R], const cplx *__restrict__ C,
                                  const uint8_t *Vs, int t, int T, int TR, const int32_t *Bt, int N, int M, int64_t nc,
                                  const double *sq, int k, int s, const int (&q)[U], const int (&l)[U], int nu,
@@ -93,9 +96,16 @@ __device__ inline void rho_batch(const int32_t *tab, const int64_t (&base)[R], c
         for (int r = 0; r < R; r++) {
             const int qq = u < nu ? q[u] : q[0], ll = u < nu ? l[u] : l[0];
             j[u][r] = 0;
-            if (act[r])
-                j[u][r] = tab ? (int64_t)tab[base[r] + ll + qq * M]
-                              : rank_lane(Vs, t + T * r, TR, Bt, N, M, nc, k, s, qq, ll);
+            if (act[r]) {
+                if (pk[r]) {
+                    const int e = packed_pos(k, s, qq < ll ? qq : ll, qq < ll ? ll : qq);
+                    j[u][r] = pk[r][(e >> 2) * n4[r] + (e & 3)];
+                } else if (tab) {
+                    j[u][r] = tab[base[r] + ll + qq * M];
+                } else {
+                    j[u][r] = rank_lane(Vs, t + T * r, TR, Bt, N, M, nc, k, s, qq, ll);
+                }
+            }
         }
     cplx g[U][R];
 #pragma unroll
@@ -130,7 +140,9 @@ __global__ __launch_bounds__(1024) void k_rho(BasisDev b, const cplx *__restrict
     extern __shared__ double smem_r[];
     const int M = b.M, N = b.N, T = blockDim.x, t = threadIdx.x, TR = T * R;
     const int P = M * (M - 1) / 2;
-    const int64_t nc = b.nc, MM = (int64_t)M * M;
+    const int64_t nc = b.nc, MM = (int64_t)M * M, nwv = (nc + 63) >> 6;
+    const int lw = t & 63;
+    const uint64_t below = lw ? (~0ull >> (64 - lw)) : 0ull;
     cplx *acc = (cplx *)smem_r;                       // nterms
     double *sq = (double *)(acc + nterms);            // N+3
     int32_t *Bt = (int32_t *)(sq + N + 3);            // M*(N+1) rank table
@@ -228,21 +240,36 @@ __global__ __launch_bounds__(1024) void k_rho(BasisDev b, const cplx *__restrict
                 }
         } else if (CLASS == 1) {
             // R3 (k < q) and R6 (q < l, k not in {q, l}), v_k >= 2
+            // packed tables (full maps): the wavefronts of a tile are the
+            // tables' wavefronts; slot offsets advance over every jump
+            const bool packed = b.packOT != nullptr;
+            const int EO = packed_entries(M, false);
             int cot[R];
+            int64_t pb[R];
 #pragma unroll
-            for (int r = 0; r < R; r++) cot[r] = 0;
+            for (int r = 0; r < R; r++) {
+                cot[r] = 0;
+                const int64_t w = i[r] >> 6;
+                pb[r] = (packed && w < nwv) ? b.poffOT[w] : 0;
+            }
             for (int k = 0; k < M; k++) {
                 bool act[R], anyact = false;
                 int64_t base[R];
                 double fk[R];
+                const int32_t *pks[R];
+                uint32_t n4[R];
 #pragma unroll
                 for (int r = 0; r < R; r++) {
                     const int vk = V(k, r);
                     act[r] = valid[r] && vk >= 2;
-                    base[r] = (act[r] && b.MapOT) ? b.strideOT[i[r]] + (int64_t)cot[r] * MM : 0;
+                    base[r] = (!packed && act[r] && b.MapOT) ? b.strideOT[i[r]] + (int64_t)cot[r] * MM : 0;
                     if (act[r]) cot[r]++;
                     fk[r] = act[r] ? sq[vk] * sq[vk - 1] : 0.0;
                     anyact = anyact || act[r];
+                    const uint64_t m = __ballot(act[r]);
+                    pks[r] = packed ? b.packOT + pb[r] + 4 * __popcll(m & below) : nullptr;
+                    n4[r] = 4u * (uint32_t)__popcll(m);
+                    pb[r] += (int64_t)__popcll(m) * EO;
                 }
                 if (!wave_any(anyact)) continue;
                 constexpr int U = RHO_U;
@@ -251,13 +278,13 @@ __global__ __launch_bounds__(1024) void k_rho(BasisDev b, const cplx *__restrict
                 for (int q = k + 1; q < M; q++) {
                     qs[nu] = q; ls[nu] = q; sl[nu] = pair_index(k, q, M);
                     if (++nu == U) {
-                        rho_batch<U, R>(b.MapOT, base, act, fk, ci, C, Vs, t, T, TR, Bt, N, M, nc, sq, k, k, qs, ls,
-                                        nu, true, acc, sl);
+                        rho_batch<U, R>(pks, n4, b.MapOT, base, act, fk, ci, C, Vs, t, T, TR, Bt, N, M, nc, sq, k, k,
+                                        qs, ls, nu, true, acc, sl);
                         nu = 0;
                     }
                 }
-                if (nu) rho_batch<U, R>(b.MapOT, base, act, fk, ci, C, Vs, t, T, TR, Bt, N, M, nc, sq, k, k, qs, ls, nu,
-                                        true, acc, sl);
+                if (nu) rho_batch<U, R>(pks, n4, b.MapOT, base, act, fk, ci, C, Vs, t, T, TR, Bt, N, M, nc, sq, k, k,
+                                        qs, ls, nu, true, acc, sl);
                 nu = 0;
                 // R6: q < l, k not in {q, l}; map entry l + q M
                 int pq = 0;
@@ -266,33 +293,46 @@ __global__ __launch_bounds__(1024) void k_rho(BasisDev b, const cplx *__restrict
                         if (q == k || l == k) continue;
                         qs[nu] = q; ls[nu] = l; sl[nu] = P + k * P + pq;
                         if (++nu == U) {
-                            rho_batch<U, R>(b.MapOT, base, act, fk, ci, C, Vs, t, T, TR, Bt, N, M, nc, sq, k, k, qs,
-                                            ls, nu, false, acc, sl);
+                            rho_batch<U, R>(pks, n4, b.MapOT, base, act, fk, ci, C, Vs, t, T, TR, Bt, N, M, nc, sq, k,
+                                            k, qs, ls, nu, false, acc, sl);
                             nu = 0;
                         }
                     }
-                if (nu) rho_batch<U, R>(b.MapOT, base, act, fk, ci, C, Vs, t, T, TR, Bt, N, M, nc, sq, k, k, qs, ls, nu,
-                                        false, acc, sl);
+                if (nu) rho_batch<U, R>(pks, n4, b.MapOT, base, act, fk, ci, C, Vs, t, T, TR, Bt, N, M, nc, sq, k, k,
+                                        qs, ls, nu, false, acc, sl);
             }
         } else {
             // R8: k < s occupied, q < l, {q, l} disjoint from {k, s}
+            const bool packed = b.packTT != nullptr;
+            const int ET = packed_entries(M, true);
             int ctt[R];
+            int64_t pb[R];
 #pragma unroll
-            for (int r = 0; r < R; r++) ctt[r] = 0;
+            for (int r = 0; r < R; r++) {
+                ctt[r] = 0;
+                const int64_t w = i[r] >> 6;
+                pb[r] = (packed && w < nwv) ? b.poffTT[w] : 0;
+            }
             int p = 0;
             for (int k = 0; k < M; k++)
                 for (int s = k + 1; s < M; s++, p++) {
                     bool act[R], anyact = false;
                     int64_t base[R];
                     double fks[R];
+                    const int32_t *pks[R];
+                    uint32_t n4[R];
 #pragma unroll
                     for (int r = 0; r < R; r++) {
                         const int vk = V(k, r), vs = V(s, r);
                         act[r] = valid[r] && vk >= 1 && vs >= 1;
-                        base[r] = (act[r] && b.MapTT) ? b.strideTT[i[r]] + (int64_t)ctt[r] * MM : 0;
+                        base[r] = (!packed && act[r] && b.MapTT) ? b.strideTT[i[r]] + (int64_t)ctt[r] * MM : 0;
                         if (act[r]) ctt[r]++;
                         fks[r] = act[r] ? sq[vk] * sq[vs] : 0.0;
                         anyact = anyact || act[r];
+                        const uint64_t m = __ballot(act[r]);
+                        pks[r] = packed ? b.packTT + pb[r] + 4 * __popcll(m & below) : nullptr;
+                        n4[r] = 4u * (uint32_t)__popcll(m);
+                        pb[r] += (int64_t)__popcll(m) * ET;
                     }
                     if (p < p0 || p >= p1) continue;
                     if (!wave_any(anyact)) continue;
@@ -305,13 +345,13 @@ __global__ __launch_bounds__(1024) void k_rho(BasisDev b, const cplx *__restrict
                             if (q == k || q == s || l == k || l == s) continue;
                             qs[nu] = l; ls[nu] = q; sl[nu] = (p - p0) * P + pq;
                             if (++nu == U) {
-                                rho_batch<U, R>(b.MapTT, base, act, fks, ci, C, Vs, t, T, TR, Bt, N, M, nc, sq, k, s,
-                                                qs, ls, nu, false, acc, sl);
+                                rho_batch<U, R>(pks, n4, b.MapTT, base, act, fks, ci, C, Vs, t, T, TR, Bt, N, M, nc, sq,
+                                                k, s, qs, ls, nu, false, acc, sl);
                                 nu = 0;
                             }
                         }
-                    if (nu) rho_batch<U, R>(b.MapTT, base, act, fks, ci, C, Vs, t, T, TR, Bt, N, M, nc, sq, k, s, qs, ls,
-                                            nu, false, acc, sl);
+                    if (nu) rho_batch<U, R>(pks, n4, b.MapTT, base, act, fks, ci, C, Vs, t, T, TR, Bt, N, M, nc, sq, k, s,
+                                            qs, ls, nu, false, acc, sl);
                 }
         }
 #undef V
@@ -356,6 +396,8 @@ static mcpi_status run_class(mcpi_basis bs, int cls, const cplx *C, int p0, int 
     // 4 for class 0 and 2 for the batched double-jump classes (4 spills)
     int R = cls == 0 ? 4 : 2;
     while (R > 1 && base_sm + (size_t)b.M * T * R > max_sm) R /= 2;
+    // and few enough that every CU gets a tile
+    while (R > 1 && (b.nc + (int64_t)T * R - 1) / ((int64_t)T * R) < ncu) R /= 2;
     const size_t sm = base_sm + (size_t)b.M * T * R;
     MCPI_ARG(sm <= max_sm, "density-matrix accumulators exceed LDS");
     int64_t tiles = (b.nc + (int64_t)T * R - 1) / ((int64_t)T * R);
