@@ -21,7 +21,7 @@ LIB = os.path.join(HERE, "libmcpi_hip.so")
 ARCH = os.environ.get("MCPI_OFFLOAD_ARCH", "gfx950")
 
 SOURCES = ["basis.hip", "acth.hip", "acth_fast_a.hip", "acth_fast_b.hip", "acth_fast_c.hip", "acth_comp.hip", "acth_multi_a.hip", "acth_multi_b.hip",
-           "acth_multi_c.hip", "rho.hip", "lanczos.hip", "batched.hip", "capi.cpp"]
+           "acth_multi_c.hip", "rho.hip", "lanczos.hip", "capi.cpp"]
 
 COMMON = [
     "-O3", "-std=c++17", "-fPIC", f"--offload-arch={ARCH}",
