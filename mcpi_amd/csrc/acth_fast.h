@@ -188,41 +188,6 @@ __device__ inline uint32_t dl(const uint32_t *Dl, int lane, int T, int a, int x)
 
 // ------------------------------------------------------- chunk bodies --
 
-// every entry q <= l of one chunk (entries whose q or l is a removed orbital
-// are no rule term: their folded coefficients are zero and their targets
-// valid states, so no branch is needed; the packed tables hold the row's
-// own index for them, a coalesced cached gather).  Row l:
-// sum_{q<l} coef r1[q] C[j] is scaled once by r1[l]; q == l uses r2[l].
-// Targets ranked from the difference tables.
-template <int M, bool RC>
-__device__ inline cplx chunk_ranked(const uint32_t (&D0)[M], const uint32_t (&D1)[M], uint32_t base,
-                                    const cplx *__restrict__ C, const RowState<M> &r, const double *sq,
-                                    const cplx *coef)
-{
-    // all targets first, then the gathers (see chunk_packed)
-    constexpr int MH = M * (M + 1) / 2;
-    uint32_t jj[MH];
-#pragma unroll
-    for (int l = 0; l < M; l++)
-#pragma unroll
-        for (int q = 0; q <= l; q++) jj[l * (l + 1) / 2 + q] = base - D0[q] - D1[l];
-    __builtin_amdgcn_sched_barrier(0);
-    cplx part = cmake(0, 0);
-#pragma unroll
-    for (int l = 0; l < M; l++) {
-        cplx pl = cmake(0, 0);
-#pragma unroll
-        for (int q = 0; q <= l; q++) {
-            const cplx cj = ld_c(C + (int64_t)jj[l * (l + 1) / 2 + q]);
-            if (q < l) pl = term<RC>(coef[q + l * M], r.r1[q], cj, pl);
-            else part = term<RC>(coef[q + l * M], r.r1[q] * sq[occ<M>(r, q) + 2], cj, part);
-        }
-        part.re = fma(r.r1[l], pl.re, part.re);
-        part.im = fma(r.r1[l], pl.im, part.im);
-    }
-    return part;
-}
-
 // Occupations of a row with the removed orbitals k and s (s == k: one) taken
 // out and the rest renumbered o' = o - [k < o] - [s < o], 8 bits each: the
 // orbital numbering of a packed slot (packed_pos in mcpi_internal.h)
@@ -305,6 +270,46 @@ __device__ inline cplx chunk_packed(const int32_t *__restrict__ tab, uint32_t of
     chunk_rows<M, RC, CP, MP, 0>(idx, C, pkc, sq, coef, part);
     return part;
 }
+// The chunk sum of one jump (k, s) with targets ranked from the row's
+// difference tables (D0, D1 over all M orbitals, base = nc - 1 - total):
+// only the rule terms, over the triangle of the MP orbitals left when k and
+// s are dropped (as in the packed slots), the tables re-indexed to that
+// numbering by a select per entry.  coef: the jump's coefficients in slot
+// order (HamCoefDev::otc / ttc).
+template <int M, bool RC, bool TT>
+__device__ inline cplx chunk_ranked(const uint32_t (&D0)[M], const uint32_t (&D1)[M], uint32_t base,
+                                    const cplx *__restrict__ C, const RowState<M> &r, const double *sq,
+                                    const cplx *coef, int k, int s)
+{
+    constexpr int MP = packed_orbitals(M, TT), G = packed_entries(M, TT) / 4;
+    // orbital of x: x below k, x + 1 between k and s, x + 2 above s (masks on
+    // the loaded values: a select between table entries would be folded
+    // into a runtime index and send the tables to scratch memory)
+    uint32_t d0[MP], d1[MP];
+#pragma unroll
+    for (int x = 0; x < MP; x++) {
+        const uint32_t lo = 0u - (uint32_t)(x < k);                          // x < k
+        const uint32_t hi = TT ? 0u - (uint32_t)(x + 1 >= s) : 0u;           // x + 2 (TT)
+        const uint32_t mid = ~lo & ~hi;                                       // x + 1
+        const uint32_t a0 = D0[x], a1 = D0[x + 1], a2 = TT ? D0[x + 2 < M ? x + 2 : M - 1] : 0u;
+        const uint32_t b0 = D1[x], b1 = D1[x + 1], b2 = TT ? D1[x + 2 < M ? x + 2 : M - 1] : 0u;
+        d0[x] = (a0 & lo) | (a1 & mid) | (a2 & hi & ~lo);
+        d1[x] = (b0 & lo) | (b1 & mid) | (b2 & hi & ~lo);
+    }
+    int32_t idx[4 * G + 4];
+#pragma unroll
+    for (int lp = 0; lp < MP; lp++)
+#pragma unroll
+        for (int qp = 0; qp <= lp; qp++) idx[lp * (lp + 1) / 2 + qp] = (int32_t)(base - d0[qp] - d1[lp]);
+    uint64_t pkc[2];
+    compress_occ(r.pk, k, s, pkc);
+    // all targets first, then the gathers (see chunk_packed)
+    __builtin_amdgcn_sched_barrier(0);
+    cplx part = cmake(0, 0);
+    chunk_rows<M, RC, const cplx *, MP, 0>(idx, C, pkc, sq, coef, part);
+    return part;
+}
+
 // the non-zero entries of one chunk from a uniform list (four at a time);
 // q and l are runtime values here, so occupations come from the packed copy,
 // square roots from the LDS table and prefix differences from the LDS copy
@@ -555,7 +560,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(SKIP ? 4 : 
                     part = chunk_packed<M, RC, false>(slot0, soff, 4u * (uint32_t)nact, C, pkc, sq,
                                                       (const cplx *)(c.otc + (int64_t)k * EO));
                 }
-                else part = chunk_ranked<M, RC>(D0, D1, base, C, r, sq, c.ot + (int64_t)k * MM);
+                else part = chunk_ranked<M, RC, false>(D0, D1, base, C, r, sq, c.otc + (int64_t)k * EO, k, k);
             } else {
                 part = chunk_list<M, DMAP, RC>(slot, 4 * nact, Dl, lane, T, base, C, r, sq, c.nz_e + c.ot_off[k],
                                                c.nz_c + c.ot_off[k], n_nz, k, k);
@@ -595,7 +600,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(SKIP ? 4 : 
                         part = chunk_packed<M, RC, true>(slot0, soff, 4u * (uint32_t)nact, C, pkc, sq,
                                                          (const cplx *)(c.ttc + (int64_t)p * ET));
                     }
-                    else part = chunk_ranked<M, RC>(D0, D1, base, C, r, sq, c.tt + (int64_t)p * MM);
+                    else part = chunk_ranked<M, RC, true>(D0, D1, base, C, r, sq, c.ttc + (int64_t)p * ET, k, s);
                 } else {
                     part = chunk_list<M, DMAP, RC>(slot, 4 * nact, Dl, lane, T, base, C, r, sq,
                                                    c.nz_e + c.tt_off[p], c.nz_c + c.tt_off[p], n_nz, k, s);
