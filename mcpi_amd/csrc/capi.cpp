@@ -363,7 +363,10 @@ std::vector<cplx> flat_ho(int M, Cmatrix Ho)
 }
 
 // device basis from caller tables, cached while the same host tables are
-// passed again (a Lanczos loop calling applyHconf_omp every step)
+// passed again (a Lanczos loop calling applyHconf_omp every step).  The
+// reference-shaped entry points that use it run one at a time (g_ref_mu,
+// held for the whole call: the cached basis must not be replaced while a
+// call still uses it).
 struct RefCache {
     mcpi_basis b = nullptr;
     int N = -1, M = -1;
@@ -371,7 +374,7 @@ struct RefCache {
     uint64_t fp = 0;
 };
 RefCache g_cache;
-std::mutex g_cache_mu;
+std::recursive_mutex g_ref_mu;
 
 uint64_t fingerprint(const int *a, int64_t n)
 {
@@ -394,7 +397,7 @@ mcpi_basis ref_basis(int N, int M, const int *IF, const int *NCmat, const int *M
                   (fingerprint(MapOT, MapOT ? n_ot * MM : 0) * 5) ^ (fingerprint(MapTT, MapTT ? n_tt * MM : 0) * 7) ^
                   (fingerprint(strideOT, strideOT ? nc : 0) * 11) ^ (fingerprint(strideTT, strideTT ? nc : 0) * 13);
     const void *key[8] = {IF, NCmat, Map, MapOT, MapTT, strideOT, strideTT, nullptr};
-    std::lock_guard<std::mutex> lk(g_cache_mu);
+    // the caller holds g_ref_mu
     if (g_cache.b && g_cache.N == N && g_cache.M == M && g_cache.fp == fp &&
         memcmp(g_cache.key, key, sizeof key) == 0)
         return g_cache.b;
@@ -535,6 +538,7 @@ Iarray OneTwoMap(int N, int M, Iarray NCmat, Iarray IF, Iarray strideC)
 void applyHconf_omp(int N, int M, Iarray Map, Iarray MapOT, Iarray MapTT, Iarray strideOT, Iarray strideTT,
                     Iarray IF, Carray C, Cmatrix Ho, Carray Hint, Carray out)
 {
+    std::lock_guard<std::recursive_mutex> lk(g_ref_mu);
     mcpi_basis b = ref_basis(N, M, IF, nullptr, Map, MapOT, MapTT, strideOT, strideTT);
     ref_apply(b, C, Ho, Hint, out);
 }
@@ -547,12 +551,14 @@ void applyHconf_XX(int N, int M, Iarray Map, Iarray MapOT, Iarray MapTT, Iarray 
 
 void applyHconf(int N, int M, Iarray NCmat, Iarray IF, Carray C, Cmatrix Ho, Carray Hint, Carray out)
 {
+    std::lock_guard<std::recursive_mutex> lk(g_ref_mu);
     mcpi_basis b = ref_basis(N, M, IF, NCmat, nullptr, nullptr, nullptr, nullptr, nullptr);
     ref_apply(b, C, Ho, Hint, out);
 }
 
 void OBrho_XM(int N, int M, Iarray Map, Iarray NCmat, Iarray IF, Carray C, Cmatrix rho)
 {
+    std::lock_guard<std::recursive_mutex> lk(g_ref_mu);
     mcpi_basis b = ref_basis(N, M, IF, NCmat, Map, nullptr, nullptr, nullptr, nullptr);
     const int64_t nc = b->d.nc;
     DevVec dC(nc);
@@ -566,6 +572,7 @@ void OBrho_XM(int N, int M, Iarray Map, Iarray NCmat, Iarray IF, Carray C, Cmatr
 void TBrho_XX(int N, int M, Iarray Map, Iarray MapOT, Iarray MapTT, Iarray strideOT, Iarray strideTT,
               Iarray NCmat, Iarray IF, Carray C, Carray rho)
 {
+    std::lock_guard<std::recursive_mutex> lk(g_ref_mu);
     mcpi_basis b = ref_basis(N, M, IF, NCmat, Map, MapOT, MapTT, strideOT, strideTT);
     const int64_t nc = b->d.nc;
     DevVec dC(nc);
@@ -576,6 +583,7 @@ void TBrho_XX(int N, int M, Iarray Map, Iarray MapOT, Iarray MapTT, Iarray strid
 int lanczos(int lm, int Npar, int Morb, Iarray IF, Iarray strideOT, Iarray strideTT, Iarray Map, Iarray MapOT,
             Iarray MapTT, Cmatrix Ho, Carray Hint, Carray diag, Carray offdiag, Cmatrix lvec)
 {
+    std::lock_guard<std::recursive_mutex> lk(g_ref_mu);
     mcpi_basis b = ref_basis(Npar, Morb, IF, nullptr, Map, MapOT, MapTT, strideOT, strideTT);
     const int64_t nc = b->d.nc;
     std::vector<cplx> ho = flat_ho(Morb, Ho);
@@ -597,6 +605,7 @@ int lanczos(int lm, int Npar, int Morb, Iarray IF, Iarray strideOT, Iarray strid
 double ground(int Niter, int Npar, int Morb, Iarray IF, Iarray strideOT, Iarray strideTT, Iarray Map,
               Iarray MapOT, Iarray MapTT, Cmatrix Ho, Carray Hint, Carray C)
 {
+    std::lock_guard<std::recursive_mutex> lk(g_ref_mu);
     mcpi_basis b = ref_basis(Npar, Morb, IF, nullptr, Map, MapOT, MapTT, strideOT, strideTT);
     const int64_t nc = b->d.nc;
     std::vector<cplx> ho = flat_ho(Morb, Ho);

@@ -29,6 +29,8 @@
 // partials, then one block per instance) and deterministic.
 #include <hip/hip_runtime.h>
 
+#include <atomic>
+
 #include <algorithm>
 #include <cmath>
 #include <cstring>
@@ -682,7 +684,7 @@ static mcpi_status launch_group(const HvGroup &g, const mcpi_ham *hams, const st
 // the caller set on the Hamiltonians (0: free).
 // kernel choice for batched groups: 0 = timed, 1 = batched specialised
 // kernel only, 2 = multi-vector kernel where the group allows it
-static int g_batched_kernel = 0;
+static std::atomic<int> g_batched_kernel{0};
 
 mcpi_status set_batched_kernel(int kind)
 {
