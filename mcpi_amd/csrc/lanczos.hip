@@ -116,6 +116,12 @@ __device__ inline cplx ldv(const LZ_GLOBAL cplx *p)
     lz_d2 t = *(const LZ_GLOBAL lz_d2 *)p;
     return cmake(t.x, t.y);
 }
+// streamed basis vectors (read once per pass): non-temporal
+__device__ inline cplx ldv_nt(const LZ_GLOBAL cplx *p)
+{
+    const lz_d2 t = __builtin_nontemporal_load((const LZ_GLOBAL lz_d2 *)p);
+    return cmake(t.x, t.y);
+}
 __device__ inline void stv(LZ_GLOBAL cplx *p, cplx z)
 {
     lz_d2 t;
@@ -337,7 +343,7 @@ __global__ __launch_bounds__(LZ_T) void lz_gemv_dots(const LanczosState *sts, co
         for (int r = 0; r < LZ_R; r++) {
             int64_t j = base + (int64_t)r * LZ_T;
             if (j < n) {
-                cplx x = ldv(vt + j);
+                cplx x = ldv_nt(vt + j);
                 re += x.re * h[r].re + x.im * h[r].im;
                 im += x.re * h[r].im - x.im * h[r].re;
             }
@@ -410,7 +416,7 @@ __global__ __launch_bounds__(LZ_T) void lz_reorth_norm(const LanczosState *sts, 
             for (int r = 0; r < LZ_RR; r++) {
                 const int64_t j = base + (int64_t)r * LZ_T;
                 if (j < n) {
-                    const cplx x = ldv(vt + j);
+                    const cplx x = ldv_nt(vt + j);
                     h[r].re -= x.re * ot.re - x.im * ot.im;
                     h[r].im -= x.re * ot.im + x.im * ot.re;
                 }
