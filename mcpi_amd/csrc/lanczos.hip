@@ -60,7 +60,7 @@ struct LzInst {
     cplx *diag, *offd;    // lm each (real parts used)
     cplx *ortho;          // lm Gram-Schmidt coefficients
     double *partial;      // 2 * nb block partials
-    double *partial_o;    // 2 * nbg * lm dot partials
+    double *partial_o;    // 2 * lm * nbg dot partials, t-major
     const double *z;      // lm Ritz coefficients (after the host solve)
     cplx *Cout;           // Ritz vector (may be null)
     int nb, nbg;          // vector and dot blocks used by this instance
@@ -360,26 +360,38 @@ __global__ __launch_bounds__(LZ_T) void lz_gemv_dots(const LanczosState *sts, co
     if (tl < t1 - t0) {
         double sa = 0, sb = 0;
         for (int x = 0; x < LZ_T / 64; x++) { sa += red[tl][x][0]; sb += red[tl][x][1]; }
-        int64_t at = 2 * ((int64_t)blockIdx.x * lm + t0 + tl);
+        int64_t at = 2 * ((int64_t)(t0 + tl) * I.nbg + blockIdx.x);
         I.partial_o[at] = sa;
         I.partial_o[at + 1] = sb;
     }
 }
 
-// 7: o_t = sum over blocks (one thread per t); grid (1, K)
-__global__ void lz_fin_ortho(const LanczosState *sts, const LzInst *in, int lm)
+// 7: o_t = sum over the nbg dot blocks.  Block (t, instance) owns one basis
+// vector; the partials of t are contiguous (t-major), each thread adds a
+// strided subset and the block reduces in a fixed order, so the result does
+// not depend on scheduling.  Grid (lm, K); blocks with t >= it + 2 exit.
+__global__ __launch_bounds__(LZ_T) void lz_fin_ortho(const LanczosState *sts, const LzInst *in)
 {
+    __shared__ double red[2][LZ_T / 64];
     const LanczosState *st = sts + blockIdx.y;
     const LzInst &I = in[blockIdx.y];
-    if (st->broke) return;
-    const int nt = st->it + 2;
-    for (int tt = threadIdx.x; tt < nt; tt += blockDim.x) {
-        double re = 0, im = 0;
-        for (int x = 0; x < I.nbg; x++) {
-            re += I.partial_o[2 * ((int64_t)x * lm + tt)];
-            im += I.partial_o[2 * ((int64_t)x * lm + tt) + 1];
-        }
-        I.ortho[tt] = cmake(re, im);
+    const int tt = blockIdx.x;
+    if (st->broke || tt >= st->it + 2) return;
+    const LZ_GLOBAL double *p = (const LZ_GLOBAL double *)I.partial_o + 2 * (int64_t)tt * I.nbg;
+    double re = 0, im = 0;
+    for (int x = threadIdx.x; x < I.nbg; x += LZ_T) {
+        re += p[2 * x];
+        im += p[2 * x + 1];
+    }
+    re = lz_wave_sum(re);
+    im = lz_wave_sum(im);
+    const int w = threadIdx.x >> 6;
+    if ((threadIdx.x & 63) == 0) { red[0][w] = re; red[1][w] = im; }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        double sa = 0, sb = 0;
+        for (int x = 0; x < LZ_T / 64; x++) { sa += red[0][x]; sb += red[1][x]; }
+        I.ortho[tt] = cmake(sa, sb);
     }
 }
 
@@ -881,6 +893,7 @@ static mcpi_status lanczos_core(int K, const mcpi_ham *hams, const cplx *const *
 
     const dim3 blk(LZ_T), gv(nbmax, K), g1(1, K);
     const dim3 gd(nbgmax, (lm + LZ_TG - 1) / LZ_TG, K);
+    const dim3 go(lm, K);
     const int ga = (K + 63) / 64;
 
     // ---- set-up: v_0, schedules, HC = H v_0, alpha_0, HC -= alpha_0 v_0
@@ -909,7 +922,7 @@ static mcpi_status lanczos_core(int K, const mcpi_ham *hams, const cplx *const *
             hipLaunchKernelGGL(lz_fin_alpha, g1, blk, 0, s, sts, cin, 0);
             hipLaunchKernelGGL(lz_sub_alpha, gv, blk, 0, s, cst, cin);
             hipLaunchKernelGGL(lz_gemv_dots, gd, blk, 0, s, cst, cin, lm);
-            hipLaunchKernelGGL(lz_fin_ortho, g1, blk, 0, s, cst, cin, lm);
+            hipLaunchKernelGGL(lz_fin_ortho, go, blk, 0, s, cst, cin);
             hipLaunchKernelGGL(lz_reorth_norm, gv, blk, 0, s, cst, cin);
             hipLaunchKernelGGL(lz_advance, dim3(ga), dim3(64), 0, s, sts, K);
             MCPI_HIP(hipGetLastError());
