@@ -3,22 +3,25 @@
 // Reference: OBrho_XM, onebodyMatrix.h:190-251 (rho1), and TBrho_XX,
 // twobodyMatrix.h:991-1487 (rho2, rules 1-8 plus symmetry copies).
 //
-// The reference loops orbital-outer / row-inner.  Here a block walks row
-// tiles (lanes = rows) and, for every representative term of a class, sums
-// the lanes' contributions with a wavefront reduction and adds the result to
-// an LDS accumulator (atomically: the wavefronts of a block add in the order
-// they arrive, so the last bits of a sum can differ from run to run).  Each
-// block writes its accumulators once; a second kernel sums the blocks in a
-// fixed order.  The host expands the representative terms into full
-// matrices with the reference's symmetry copies.
+// The reference loops orbital-outer / row-inner.  Here every class of
+// representative terms is summed into LDS accumulators of a block
+// (atomically: the wavefronts of a block add in the order they arrive, so
+// the last bits of a sum can differ from run to run); each block writes its
+// accumulators once and a second kernel sums the blocks in a fixed order.
+// The host expands the representative terms into full matrices with the
+// reference's symmetry copies.  Single jumps and the diagonal (class 0):
+// lanes are rows and each term is a wavefront reduction.  Double jumps
+// (classes 1, 2): lanes are terms and a wavefront steps through the rows
+// where a jump is active, so no reduction is needed.
 //
-// Targets come from the stored maps when the basis has them and are ranked
-// on the fly otherwise (closed-form FockToIndex over the lane's prefix
-// counts, configurationsMap.h:273-305), so the adjoint also runs on bases
-// that store only Map, or no map at all.
+// Single-jump targets come from Map when the basis has it and are ranked on
+// the fly otherwise (closed-form FockToIndex over the row's prefix counts,
+// configurationsMap.h:273-305); double-jump targets are ranked from per-row
+// difference tables (M <= 16) or read from the maps, so the adjoint also
+// runs on bases that store only Map, or no map at all.
 //
 // Term classes (compact layouts, P = M(M-1)/2 pairs, pq = pair index):
-//   class 0 (single jumps and diagonal):
+//   class 0 (single jumps and diagonal; class 3: r1d and r1o only):
 //     r1d[k] | r1o[kl] | R1[k] | R2[ks] | R4[kl] | R5[ks] | R7[kl][s]
 //   class 1 (same-orbital double jumps):  R3[kq] | R6[k][ql]
 //   class 2 (two-orbital double jumps):   R8[ks - p0][ql]   (pairs p0..p1-1)
@@ -57,8 +60,56 @@ __device__ inline void wave_add(cplx z, cplx *acc, int idx, bool any)
 
 __device__ inline bool wave_any(bool p) { return __ballot(p) != 0; }
 
+// One exchange step of the batched reduction: of its 2H values the lane keeps
+// the half selected by lane bit `mask` (upper half if set), adds its
+// partner's copy of that half and passes the other half to the partner.
+template <int H>
+__device__ inline void xchg_step(cplx (&z)[8], int mask)
+{
+    const bool hi = (threadIdx.x & mask) != 0;
+#pragma unroll
+    for (int u = 0; u < H; u++) {
+        const cplx keep = hi ? z[u + H] : z[u];
+        const cplx send = hi ? z[u] : z[u + H];
+        z[u].re = keep.re + __shfl_xor(send.re, mask, 64);
+        z[u].im = keep.im + __shfl_xor(send.im, mask, 64);
+    }
+}
+
+// Wavefront totals of 8 per-lane values z[0..7], added to acc[slot[u]] for
+// the u in the uniform bit mask `use`.  Three exchange steps over lane bits
+// 0, 1, 2 leave lane L with a partial of term u(L) = 4 b0 + 2 b1 + b2 over
+// the 8 lanes L ^ x, x < 8; three butterfly steps over lane bits 3, 4, 5
+// complete it in every lane, and lanes 0..7 (one per term) add the totals:
+// 10 complex shuffles and one LDS atomic pair for the 8 terms instead of 48
+// shuffles and 8 atomic pairs one after another.
+__device__ inline void wave_add8(cplx (&z)[8], cplx *acc, const int (&slot)[8], uint32_t use)
+{
+    xchg_step<4>(z, 1);
+    xchg_step<2>(z, 2);
+    xchg_step<1>(z, 4);
+    double re = z[0].re, im = z[0].im;
+#pragma unroll
+    for (int o = 8; o < 64; o <<= 1) {
+        re += __shfl_xor(re, o, 64);
+        im += __shfl_xor(im, o, 64);
+    }
+    const int lane = threadIdx.x & 63;
+    const int u = ((lane & 1) << 2) | (lane & 2) | ((lane >> 2) & 1);
+    // slot[u] by masks: a select chain would be folded into a runtime index
+    // and send the slot array to scratch memory
+    int sl = 0;
+#pragma unroll
+    for (int x = 0; x < 8; x++) sl |= slot[x] & (0 - (int)(u == x));
+    if (lane < 8 && ((use >> u) & 1u)) {
+        atomicAdd(&acc[sl].re, re);
+        atomicAdd(&acc[sl].im, im);
+    }
+}
+
 // rank of this lane's row with particles removed from a, b2 (b2 = -1: none)
 // and added to c, d (d = -1: none); occupations from the LDS tile
+// ([orbital][row of tile], T rows)
 __device__ inline int64_t rank_lane(const uint8_t *Vs, int t, int T, const int32_t *Bt, int N, int M, int64_t nc,
                                     int a, int b2, int c, int d)
 {
@@ -72,77 +123,124 @@ __device__ inline int64_t rank_lane(const uint8_t *Vs, int t, int T, const int32
     return nc - 1 - s;
 }
 
-// U terms (q[u], l[u]) of one double jump (k, s), rows r < R of the lane:
-// every target first (map entries or ranks), then every gather, then the
-// sums and U wavefront reductions, so that the memory accesses of the batch
-// are in flight together instead of one dependent chain per term.
-// Targets: packed (pk[r] != nullptr: the row's slot in the wave-packed
-// tables, entry packed_pos(k, s, q, l) at group stride n4[r]), else tab
-// (MapOT or MapTT at the row's chunk offset base[r]), else ranked on the
-// fly.  f[r]: the jump's factor (0 for rows without the jump); diag: q == l
-// terms (sqrt((v+1)(v+2)) instead of sqrt(v_q+1) sqrt(v_l+1)).
-template <int U, int R>
-__device__ inline void rho_batch(const int32_t *const (&pk)[R], const uint32_t (&n4)[R], const int32_t *tab,
-                                 const int64_t (&base)[R], const bool (&act)[R],
-                                 const double (&f)[R], const cplx (&ci)[R], const cplx *__restrict__ C,
-                                 const uint8_t *Vs, int t, int T, int TR, const int32_t *Bt, int N, int M, int64_t nc,
-                                 const double *sq, int k, int s, const int (&q)[U], const int (&l)[U], int nu,
-                                 bool diag, cplx *acc, const int (&slot)[U])
+// Terms of a double jump in lane order (lane-per-term kernels below).
+// Class 2, jump (k, s), k < s: the pairs q < l of the M - 2 orbitals other
+// than k and s, term u = l'(l'-1)/2 + q' in the renumbering of packed_pos;
+// output (p - p0) P + pair(q, l).  Class 1, jump k: first the M - 1 - k
+// pairs (q, q), q = k + 1 + u, output pair(k, q); then the pairs q < l of
+// the M - 1 orbitals other than k, output P + k P + pair(q, l).  e: entry of
+// the packed slot; tix: entry of the row's M x M block of the
+// reference-layout map (q + l M for class 2, l + q M for class 1).
+struct JumpTerm {
+    int q, l, slot, e, tix;
+    bool diag, ok;
+};
+
+// (q', l'), q' < l', of the strict-triangle index u = l'(l'-1)/2 + q'
+__device__ inline void strict_tri(int u, int &qp, int &lp)
 {
-    int64_t j[U][R];
-#pragma unroll
-    for (int u = 0; u < U; u++)
-#pragma unroll
-        for (int r = 0; r < R; r++) {
-            const int qq = u < nu ? q[u] : q[0], ll = u < nu ? l[u] : l[0];
-            j[u][r] = 0;
-            if (act[r]) {
-                if (pk[r]) {
-                    const int e = packed_pos(k, s, qq < ll ? qq : ll, qq < ll ? ll : qq);
-                    j[u][r] = pk[r][(e >> 2) * n4[r] + (e & 3)];
-                } else if (tab) {
-                    j[u][r] = tab[base[r] + ll + qq * M];
-                } else {
-                    j[u][r] = rank_lane(Vs, t + T * r, TR, Bt, N, M, nc, k, s, qq, ll);
-                }
-            }
-        }
-    cplx g[U][R];
-#pragma unroll
-    for (int u = 0; u < U; u++)
-#pragma unroll
-        for (int r = 0; r < R; r++) g[u][r] = act[r] ? ldcr(C + j[u][r]) : cmake(0, 0);
-#pragma unroll
-    for (int u = 0; u < U; u++) {
-        if (u >= nu) break;   // uniform
-        cplx z = cmake(0, 0);
-#pragma unroll
-        for (int r = 0; r < R; r++) {
-            const int vq = Vs[q[u] * TR + t + T * r], vl = Vs[l[u] * TR + t + T * r];
-            const double x = diag ? sq[vq + 1] * sq[vq + 2] : sq[vq + 1] * sq[vl + 1];
-            z = cfma(cfma_conj(ci[r], g[u][r], cmake(0, 0)), cmake(f[r] * x, 0), z);
-        }
-        wave_add(z, acc, slot[u], true);
-    }
+    int L = (int)((1.0 + sqrt(1.0 + 8.0 * (double)u)) * 0.5);
+    while (L * (L - 1) / 2 > u) L--;
+    while ((L + 1) * L / 2 <= u) L++;
+    lp = L;
+    qp = u - L * (L - 1) / 2;
 }
 
-// A block of T = 1024 threads (16 wavefronts: the accumulators, up to
-// 120 KB of LDS, allow one block per CU, so the block has to fill the CU)
-// walks tiles of T*R rows; lane t holds rows tile + t + T*r, r < R, and sums
-// its R rows' contributions to a term before the wavefront reduction, which
-// divides the number of reductions and LDS atomics by R.
-#define RHO_U 8   // terms per batch in classes 1 and 2
+__device__ inline int jump_terms(int cls, int M, int k)
+{
+    const int MP = cls == 2 ? M - 2 : M - 1;
+    return (cls == 2 ? 0 : M - 1 - k) + (MP > 1 ? MP * (MP - 1) / 2 : 0);
+}
 
-template <int CLASS, int R>
-__global__ __launch_bounds__(1024) void k_rho(BasisDev b, const cplx *__restrict__ C, int p0, int p1,
-                                             int nterms, cplx *__restrict__ partial)
+template <int CLASS>
+__device__ inline JumpTerm jump_term(int M, int P, int k, int s, int p, int p0, int u)
+{
+    JumpTerm J;
+    J.ok = u < jump_terms(CLASS, M, k);
+    int qp, lp;
+    if (CLASS == 2) {
+        strict_tri(u, qp, lp);
+        J.q = packed_orbital(k, s, qp);
+        J.l = packed_orbital(k, s, lp);
+        J.diag = false;
+        J.slot = (p - p0) * P + (J.ok ? pair_index(J.q, J.l, M) : 0);
+        J.tix = J.q + J.l * M;
+    } else {
+        const int n3 = M - 1 - k;
+        if (u < n3) {
+            J.q = J.l = k + 1 + u;
+            J.diag = true;
+            J.slot = pair_index(k, J.q, M);
+        } else {
+            strict_tri(u - n3, qp, lp);
+            J.q = packed_orbital(k, k, qp);
+            J.l = packed_orbital(k, k, lp);
+            J.diag = false;
+            J.slot = J.ok ? P + k * P + pair_index(J.q, J.l, M) : 0;
+        }
+        J.tix = J.l + J.q * M;
+    }
+    if (!J.ok) J.q = J.l = J.slot = J.tix = 0;
+    J.e = J.ok ? packed_pos(k, s, J.q, J.l) : 0;
+    return J;
+}
+
+// rank of a row (occupations row[0..M-1]) with particles removed from a and
+// b2 and added to c and d (configurationsMap.h:273-305, closed form)
+__device__ inline int64_t rank_row(const uint8_t *row, const int32_t *Bt, int N, int M, int64_t nc, int a, int b2,
+                                   int c, int d)
+{
+    int P = 0;
+    int64_t s = 0;
+    for (int j = 1; j < M; j++) {
+        P += row[j - 1];
+        const int p = P - (a < j) - (b2 < j) + (c < j) + (d < j);
+        s += Bt[j * (N + 1) + p];
+    }
+    return nc - 1 - s;
+}
+
+// inclusive prefix sum over each row of 16 lanes (DPP row shifts; lanes
+// shifted in from outside the row read 0)
+__device__ inline uint32_t row_scan16(uint32_t v)
+{
+    v += (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0x111, 0xf, 0xf, true);   // row_shr:1
+    v += (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0x112, 0xf, 0xf, true);   // row_shr:2
+    v += (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0x114, 0xf, 0xf, true);   // row_shr:4
+    v += (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0x118, 0xf, 0xf, true);   // row_shr:8
+    return v;
+}
+
+__device__ inline double readlane_d(double x, int lane)
+{
+    const uint64_t u = __builtin_bit_cast(uint64_t, x);
+    const uint32_t lo = (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)u, lane);
+    const uint32_t hi = (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)(u >> 32), lane);
+    return __builtin_bit_cast(double, ((uint64_t)hi << 32) | lo);
+}
+
+__device__ inline int64_t readlane_l(int64_t x, int lane)
+{
+    const uint32_t lo = (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)(uint64_t)x, lane);
+    const uint32_t hi = (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)((uint64_t)x >> 32), lane);
+    return (int64_t)(((uint64_t)hi << 32) | lo);
+}
+
+// Class 0.  A block of T = 1024 threads walks tiles of T*R rows (lanes =
+// rows); lane t holds rows tile + t + T*r, r < R, and sums its R rows'
+// contributions to a term before the wavefront reduction, which divides the
+// number of reductions and LDS atomics by R.  ALL: every class-0 term (the
+// two-body density); else r1d and r1o only (the one-body density).  With
+// ALL the terms of a pair (k, l) (R2, r1o, R4, R5 and the M - 2 terms of
+// R7) are reduced 8 at a time (wave_add8).
+template <bool ALL, int R>
+__global__ __launch_bounds__(1024) void k_rho(BasisDev b, const cplx *__restrict__ C, int nterms,
+                                             cplx *__restrict__ partial)
 {
     extern __shared__ double smem_r[];
     const int M = b.M, N = b.N, T = blockDim.x, t = threadIdx.x, TR = T * R;
     const int P = M * (M - 1) / 2;
-    const int64_t nc = b.nc, MM = (int64_t)M * M, nwv = (nc + 63) >> 6;
-    const int lw = t & 63;
-    const uint64_t below = lw ? (~0ull >> (64 - lw)) : 0ull;
+    const int64_t nc = b.nc;
     cplx *acc = (cplx *)smem_r;                       // nterms
     double *sq = (double *)(acc + nterms);            // N+3
     int32_t *Bt = (int32_t *)(sq + N + 3);            // M*(N+1) rank table
@@ -167,8 +265,7 @@ __global__ __launch_bounds__(1024) void k_rho(BasisDev b, const cplx *__restrict
         }
         __syncthreads();
 #define V(j, r) ((int)Vs[(j) * TR + t + T * (r)])
-        if (CLASS == 0) {
-            int o = 0;
+        {
             // r1d, R1
             for (int k = 0; k < M; k++) {
                 double a = 0, a2 = 0;
@@ -180,30 +277,26 @@ __global__ __launch_bounds__(1024) void k_rho(BasisDev b, const cplx *__restrict
                     a2 += m2[r] * vk * (vk - 1);
                     any2 = any2 || vk >= 2;
                 }
-                wave_add(cmake(a, 0), acc, o + k, true);
-                wave_add(cmake(a2, 0), acc, o + P + M + k, wave_any(any2));
+                wave_add(cmake(a, 0), acc, k, true);
+                if (ALL) wave_add(cmake(a2, 0), acc, M + P + k, wave_any(any2));
             }
-            o += M;  // r1o at o
-            const int o_r1o = o, o_R2 = M + P + M, o_R4 = o_R2 + P, o_R5 = o_R4 + P, o_R7 = o_R5 + P;
+            const int o_r1o = M, o_R2 = M + P + M, o_R4 = o_R2 + P, o_R5 = o_R4 + P, o_R7 = o_R5 + P;
             int pq = 0;
             for (int k = 0; k < M; k++)
                 for (int l = k + 1; l < M; l++, pq++) {
                     // R2: |C|^2 v_k v_l over all rows
-                    double a = 0;
-                    bool any = false, anyact = false;
+                    double a2 = 0;
+                    bool anyact = false;
 #pragma unroll
                     for (int r = 0; r < R; r++) {
                         const int vk = V(k, r), vl = V(l, r);
-                        a += m2[r] * vk * vl;
-                        any = any || (vk && vl);
+                        a2 += m2[r] * vk * vl;
                         anyact = anyact || (valid[r] && vk >= 1);
                     }
-                    wave_add(cmake(a, 0), acc, o_R2 + pq, wave_any(any));
-                    if (!wave_any(anyact)) continue;
+                    if (!wave_any(anyact)) continue;   // no row with v_k >= 1: every term is zero
                     cplx g[R];
                     double root[R];
                     cplx z1 = cmake(0, 0), z4 = cmake(0, 0), z5 = cmake(0, 0);
-                    bool any4 = false, any5 = false;
 #pragma unroll
                     for (int r = 0; r < R; r++) {
                         const int vk = V(k, r), vl = V(l, r);
@@ -216,147 +309,228 @@ __global__ __launch_bounds__(1024) void k_rho(BasisDev b, const cplx *__restrict
                         }
                         root[r] = act ? sq[vk] * sq[vl + 1] : 0.0;
                         z1 = cfma(g[r], cmake(root[r], 0), z1);
-                        z4 = cfma(g[r], cmake(act && vk >= 2 ? (vk - 1) * root[r] : 0.0, 0), z4);
-                        z5 = cfma(g[r], cmake(act && vl >= 1 ? vl * root[r] : 0.0, 0), z5);
-                        any4 = any4 || (act && vk >= 2);
-                        any5 = any5 || (act && vl >= 1);
-                    }
-                    wave_add(z1, acc, o_r1o + pq, true);
-                    wave_add(z4, acc, o_R4 + pq, wave_any(any4));
-                    wave_add(z5, acc, o_R5 + pq, wave_any(any5));
-                    for (int s = 0; s < M; s++) {
-                        if (s == k || s == l) continue;
-                        cplx z7 = cmake(0, 0);
-                        bool any7 = false;
-#pragma unroll
-                        for (int r = 0; r < R; r++) {
-                            const int vs = V(s, r);
-                            const bool act = valid[r] && V(k, r) >= 1;
-                            z7 = cfma(g[r], cmake(act && vs >= 1 ? vs * root[r] : 0.0, 0), z7);
-                            any7 = any7 || (act && vs >= 1);
-                        }
-                        wave_add(z7, acc, o_R7 + pq * M + s, wave_any(any7));
-                    }
-                }
-        } else if (CLASS == 1) {
-            // R3 (k < q) and R6 (q < l, k not in {q, l}), v_k >= 2
-            // packed tables (full maps): the wavefronts of a tile are the
-            // tables' wavefronts; slot offsets advance over every jump
-            const bool packed = b.packOT != nullptr;
-            const int EO = packed_entries(M, false);
-            int cot[R];
-            int64_t pb[R];
-#pragma unroll
-            for (int r = 0; r < R; r++) {
-                cot[r] = 0;
-                const int64_t w = i[r] >> 6;
-                pb[r] = (packed && w < nwv) ? b.poffOT[w] : 0;
-            }
-            for (int k = 0; k < M; k++) {
-                bool act[R], anyact = false;
-                int64_t base[R];
-                double fk[R];
-                const int32_t *pks[R];
-                uint32_t n4[R];
-#pragma unroll
-                for (int r = 0; r < R; r++) {
-                    const int vk = V(k, r);
-                    act[r] = valid[r] && vk >= 2;
-                    base[r] = (!packed && act[r] && b.MapOT) ? b.strideOT[i[r]] + (int64_t)cot[r] * MM : 0;
-                    if (act[r]) cot[r]++;
-                    fk[r] = act[r] ? sq[vk] * sq[vk - 1] : 0.0;
-                    anyact = anyact || act[r];
-                    const uint64_t m = __ballot(act[r]);
-                    pks[r] = packed ? b.packOT + pb[r] + 4 * __popcll(m & below) : nullptr;
-                    n4[r] = 4u * (uint32_t)__popcll(m);
-                    pb[r] += (int64_t)__popcll(m) * EO;
-                }
-                if (!wave_any(anyact)) continue;
-                constexpr int U = RHO_U;
-                int qs[U], ls[U], sl[U], nu = 0;
-                // R3: q == l (k < q); the map entry q + q M is read as l + q M with l = q
-                for (int q = k + 1; q < M; q++) {
-                    qs[nu] = q; ls[nu] = q; sl[nu] = pair_index(k, q, M);
-                    if (++nu == U) {
-                        rho_batch<U, R>(pks, n4, b.MapOT, base, act, fk, ci, C, Vs, t, T, TR, Bt, N, M, nc, sq, k, k,
-                                        qs, ls, nu, true, acc, sl);
-                        nu = 0;
-                    }
-                }
-                if (nu) rho_batch<U, R>(pks, n4, b.MapOT, base, act, fk, ci, C, Vs, t, T, TR, Bt, N, M, nc, sq, k, k,
-                                        qs, ls, nu, true, acc, sl);
-                nu = 0;
-                // R6: q < l, k not in {q, l}; map entry l + q M
-                int pq = 0;
-                for (int q = 0; q < M; q++)
-                    for (int l = q + 1; l < M; l++, pq++) {
-                        if (q == k || l == k) continue;
-                        qs[nu] = q; ls[nu] = l; sl[nu] = P + k * P + pq;
-                        if (++nu == U) {
-                            rho_batch<U, R>(pks, n4, b.MapOT, base, act, fk, ci, C, Vs, t, T, TR, Bt, N, M, nc, sq, k,
-                                            k, qs, ls, nu, false, acc, sl);
-                            nu = 0;
+                        if (ALL) {
+                            z4 = cfma(g[r], cmake(act && vk >= 2 ? (vk - 1) * root[r] : 0.0, 0), z4);
+                            z5 = cfma(g[r], cmake(act && vl >= 1 ? vl * root[r] : 0.0, 0), z5);
                         }
                     }
-                if (nu) rho_batch<U, R>(pks, n4, b.MapOT, base, act, fk, ci, C, Vs, t, T, TR, Bt, N, M, nc, sq, k, k,
-                                        qs, ls, nu, false, acc, sl);
-            }
-        } else {
-            // R8: k < s occupied, q < l, {q, l} disjoint from {k, s}
-            const bool packed = b.packTT != nullptr;
-            const int ET = packed_entries(M, true);
-            int ctt[R];
-            int64_t pb[R];
-#pragma unroll
-            for (int r = 0; r < R; r++) {
-                ctt[r] = 0;
-                const int64_t w = i[r] >> 6;
-                pb[r] = (packed && w < nwv) ? b.poffTT[w] : 0;
-            }
-            int p = 0;
-            for (int k = 0; k < M; k++)
-                for (int s = k + 1; s < M; s++, p++) {
-                    bool act[R], anyact = false;
-                    int64_t base[R];
-                    double fks[R];
-                    const int32_t *pks[R];
-                    uint32_t n4[R];
-#pragma unroll
-                    for (int r = 0; r < R; r++) {
-                        const int vk = V(k, r), vs = V(s, r);
-                        act[r] = valid[r] && vk >= 1 && vs >= 1;
-                        base[r] = (!packed && act[r] && b.MapTT) ? b.strideTT[i[r]] + (int64_t)ctt[r] * MM : 0;
-                        if (act[r]) ctt[r]++;
-                        fks[r] = act[r] ? sq[vk] * sq[vs] : 0.0;
-                        anyact = anyact || act[r];
-                        const uint64_t m = __ballot(act[r]);
-                        pks[r] = packed ? b.packTT + pb[r] + 4 * __popcll(m & below) : nullptr;
-                        n4[r] = 4u * (uint32_t)__popcll(m);
-                        pb[r] += (int64_t)__popcll(m) * ET;
+                    if (!ALL) {
+                        wave_add(z1, acc, o_r1o + pq, true);
+                        continue;
                     }
-                    if (p < p0 || p >= p1) continue;
-                    if (!wave_any(anyact)) continue;
-                    constexpr int U = RHO_U;
-                    int qs[U], ls[U], sl[U], nu = 0;
-                    int pq = 0;
-                    // MapTT entry q + l M (the transposed entry l + q M holds the same target)
-                    for (int q = 0; q < M; q++)
-                        for (int l = q + 1; l < M; l++, pq++) {
-                            if (q == k || q == s || l == k || l == s) continue;
-                            qs[nu] = l; ls[nu] = q; sl[nu] = (p - p0) * P + pq;
-                            if (++nu == U) {
-                                rho_batch<U, R>(pks, n4, b.MapTT, base, act, fks, ci, C, Vs, t, T, TR, Bt, N, M, nc, sq,
-                                                k, s, qs, ls, nu, false, acc, sl);
-                                nu = 0;
+                    // terms x of the pair: R2, r1o, R4, R5, then R7 for the
+                    // x - 4'th orbital s other than k and l
+                    const int nx = 4 + (M - 2);
+                    for (int x0 = 0; x0 < nx; x0 += 8) {
+                        cplx zb[8];
+                        int sb[8];
+                        uint32_t use = 0;
+#pragma unroll
+                        for (int u = 0; u < 8; u++) {
+                            const int x = x0 + u;   // uniform
+                            zb[u] = cmake(0, 0);
+                            sb[u] = 0;
+                            if (x >= nx) continue;
+                            use |= 1u << u;
+                            if (x == 0) { zb[u] = cmake(a2, 0); sb[u] = o_R2 + pq; }
+                            else if (x == 1) { zb[u] = z1; sb[u] = o_r1o + pq; }
+                            else if (x == 2) { zb[u] = z4; sb[u] = o_R4 + pq; }
+                            else if (x == 3) { zb[u] = z5; sb[u] = o_R5 + pq; }
+                            else {
+                                int so = x - 4;
+                                so += (so >= k);
+                                so += (so >= l);
+#pragma unroll
+                                for (int r = 0; r < R; r++) {
+                                    const int vs = V(so, r);
+                                    const bool act = valid[r] && V(k, r) >= 1;
+                                    zb[u] = cfma(g[r], cmake(act && vs >= 1 ? vs * root[r] : 0.0, 0), zb[u]);
+                                }
+                                sb[u] = o_R7 + pq * M + so;
                             }
                         }
-                    if (nu) rho_batch<U, R>(pks, n4, b.MapTT, base, act, fks, ci, C, Vs, t, T, TR, Bt, N, M, nc, sq, k, s,
-                                            qs, ls, nu, false, acc, sl);
+                        wave_add8(zb, acc, sb, use);
+                    }
                 }
         }
 #undef V
         __syncthreads();
     }
+    for (int x = t; x < nterms; x += T) partial[(int64_t)blockIdx.x * nterms + x] = acc[x];
+}
+
+// Classes 1 and 2, lane per term.  A wavefront owns 64 consecutive rows (one
+// wavefront of the packed tables, whose slot order it follows) and walks the
+// jumps; for each jump it steps through the rows where the jump is active,
+// RHO_G at a time, and lane u evaluates term u of the jump (jump_term) on
+// each of them, summing in registers.  One LDS atomic pair per lane and jump
+// adds the wavefront's sums to the block's accumulators: no reductions
+// across lanes, and each load instruction carries up to 64 useful lanes
+// (with rows as lanes only the rows where the jump is active would be
+// useful, about a quarter of them at N = M = 12).  Targets ranked on the
+// fly (mode 2), or for M > 16 from the packed tables (mode 0) or the
+// reference-layout maps (1) when the basis has them.
+// The row's amplitude is broadcast with readlane; its occupations are read
+// from the wavefront's LDS rows ([row][orbital], MS bytes a row).
+#define RHO_G 8
+
+template <int CLASS>
+__global__ __launch_bounds__(1024) void k_rho_jumps(BasisDev b, const cplx *__restrict__ C, int p0, int p1,
+                                                   int nterms, cplx *__restrict__ partial)
+{
+    static_assert(CLASS == 1 || CLASS == 2, "double-jump classes");
+    constexpr bool TT = CLASS == 2;
+    extern __shared__ double smem_r[];
+    const int M = b.M, N = b.N, T = blockDim.x, t = threadIdx.x;
+    const int MS = (M + 3) & ~3;
+    const int P = M * (M - 1) / 2;
+    const int64_t nc = b.nc, MM = (int64_t)M * M, nwv = (nc + 63) >> 6;
+    const int lane = t & 63, wl = t >> 6, nwb = T >> 6;
+    cplx *acc = (cplx *)smem_r;                                              // nterms
+    double *sq = (double *)(acc + nterms);                                   // N+3
+    int32_t *Bt = (int32_t *)(sq + N + 3);                                   // M*(N+1) rank table
+    uint8_t *Vs = (uint8_t *)(Bt + M * (N + 1)) + (size_t)wl * 64 * MS;     // this wavefront's rows
+    uint8_t *PF = (uint8_t *)(Bt + M * (N + 1)) + (size_t)(nwb + wl) * 64 * MS;  // their prefix counts
+    for (int x = t; x < nterms; x += T) acc[x] = cmake(0, 0);
+    for (int x = t; x < N + 3; x += T) sq[x] = sqrt((double)x);
+    for (int x = t; x < M * (N + 1); x += T) Bt[x] = b.B[x];
+    __syncthreads();
+
+    const int32_t *pack = TT ? b.packTT : b.packOT;
+    const int64_t *poff = TT ? b.poffTT : b.poffOT;
+    const int32_t *tab = TT ? b.MapTT : b.MapOT;
+    const int64_t *stride = TT ? b.strideTT : b.strideOT;
+    // M <= 16: targets ranked from difference tables built once per row and
+    // jump (one row of 16 lanes per table), which is cheaper than reading
+    // them from the maps (N = M = 12: 7.6 against 10.7 ms for rho2); else
+    // from the packed tables, the reference-layout maps or rank_row per term
+    const bool dtab = M <= 16;
+    const int mode = dtab ? 2 : (pack ? 0 : (tab ? 1 : 2));
+    const int E = packed_entries(M, TT);
+    const uint32_t top = (uint32_t)(nc - 1);
+
+    for (int64_t w = (int64_t)blockIdx.x * nwb + wl; w < nwv; w += (int64_t)gridDim.x * nwb) {
+        const int64_t i = w * 64 + lane;
+        const bool valid = i < nc;
+        uint8_t *own = Vs + lane * MS;
+        {
+            int pc = 0;
+            for (int j = 0; j < M; j++) {
+                const int v = valid ? b.IF[i * M + j] : 0;
+                own[j] = (uint8_t)v;
+                PF[lane * MS + j] = (uint8_t)pc;   // particles in orbitals below j
+                pc += v;
+            }
+        }
+        __builtin_amdgcn_wave_barrier();
+        const cplx ci = valid ? ldcr(C + i) : cmake(0, 0);
+        int64_t pb = mode == 0 ? poff[w] : 0;                       // the wavefront's slots
+        int64_t rbase = (mode == 1 && valid) ? stride[i] : 0;       // the row's map blocks
+        int p = 0;
+        for (int k = 0; k < M; k++) {
+            for (int s = TT ? k + 1 : k; s < (TT ? M : k + 1); s++, p++) {
+                const int vk = own[k], vs = own[s];
+                const bool act = valid && (TT ? (vk >= 1 && vs >= 1) : vk >= 2);
+                const uint64_t m = __ballot(act);
+                const int nact = __popcll(m);
+                const int64_t jpb = pb;
+                pb += (int64_t)nact * E;
+                const int64_t mybase = rbase;
+                if (mode == 1 && act) rbase += MM;
+                if (TT && (p < p0 || p >= p1)) continue;
+                if (m == 0) continue;
+                const int nt = jump_terms(CLASS, M, k);
+                for (int c0 = 0; c0 < nt; c0 += 64) {
+                    const JumpTerm J = jump_term<CLASS>(M, P, k, s, p, p0, c0 + lane);
+                    double zre = 0, zim = 0;
+                    uint64_t rem = m;
+                    int r0 = 0;   // active rows before the group (slot order)
+                    while (rem) {
+                        int a[RHO_G], ng = 0;
+#pragma unroll
+                        for (int g = 0; g < RHO_G; g++) {
+                            a[g] = 0;
+                            if (rem) {
+                                a[g] = __builtin_ctzll(rem);
+                                rem &= rem - 1;
+                                ng++;
+                            }
+                        }
+                        int64_t j[RHO_G];
+                        if (dtab) {
+                            // rows 4h .. 4h+3 of the group: lane 16 x + o builds entry o of
+                            // the tables of row 4h + x (removal profile k, s); the target
+                            // of added q <= l is top - tot - D0[q] - D1[l]
+#pragma unroll
+                            for (int h = 0; h < RHO_G / 4; h++) {
+                                const int x = lane >> 4, o = lane & 15;
+                                int ar = a[4 * h];
+#pragma unroll
+                                for (int y = 1; y < 4; y++) ar = (x == y) ? a[4 * h + y] : ar;
+                                uint32_t d0 = 0, d1 = 0, c2 = 0;
+                                if (4 * h + x < ng && o >= 1 && o < M) {
+                                    const int pp = PF[ar * MS + o] - (k < o) - (s < o);
+                                    const int32_t *bt = Bt + o * (N + 1);
+                                    const uint32_t c0 = (uint32_t)bt[min(max(pp, 0), N)];
+                                    const uint32_t c1 = (uint32_t)bt[min(max(pp + 1, 0), N)];
+                                    c2 = (uint32_t)bt[min(max(pp + 2, 0), N)];
+                                    d0 = c0 - c1;
+                                    d1 = c1 - c2;
+                                }
+                                d0 = row_scan16(d0);
+                                d1 = row_scan16(d1);
+                                c2 = row_scan16(c2);
+#pragma unroll
+                                for (int y = 0; y < 4; y++) {
+                                    const int g = 4 * h + y;
+                                    const uint32_t tot = (uint32_t)__builtin_amdgcn_readlane((int)c2, 16 * y + M - 1);
+                                    const uint32_t D0 = (uint32_t)__shfl((int)d0, 16 * y + J.q, 64);
+                                    const uint32_t D1 = (uint32_t)__shfl((int)d1, 16 * y + J.l, 64);
+                                    j[g] = (g < ng && J.ok) ? (int64_t)(top - tot - D0 - D1) : 0;
+                                }
+                            }
+                        } else {
+#pragma unroll
+                            for (int g = 0; g < RHO_G; g++) {
+                                j[g] = 0;
+                                if (g < ng && J.ok) {
+                                    if (mode == 0)
+                                        j[g] = pack[jpb + (int64_t)(J.e >> 2) * 4 * nact + 4 * (r0 + g) + (J.e & 3)];
+                                    else if (mode == 1)
+                                        j[g] = tab[readlane_l(mybase, a[g]) + J.tix];
+                                    else
+                                        j[g] = rank_row(Vs + a[g] * MS, Bt, N, M, nc, k, s, J.q, J.l);
+                                }
+                            }
+                        }
+                        cplx cj[RHO_G];
+#pragma unroll
+                        for (int g = 0; g < RHO_G; g++) cj[g] = (g < ng && J.ok) ? ldcr(C + j[g]) : cmake(0, 0);
+#pragma unroll
+                        for (int g = 0; g < RHO_G; g++) {
+                            if (g >= ng) break;   // uniform
+                            const cplx cr = cmake(readlane_d(ci.re, a[g]), readlane_d(ci.im, a[g]));
+                            const uint8_t *row = Vs + a[g] * MS;
+                            const int rk = row[k], rs = row[s];
+                            const double f = TT ? sq[rk] * sq[rs] : sq[rk] * sq[rk - 1];
+                            const int vq = row[J.q], vl = row[J.l];
+                            const double x = f * (J.diag ? sq[vq + 1] * sq[vq + 2] : sq[vq + 1] * sq[vl + 1]);
+                            const cplx z = cfma_conj(cr, cj[g], cmake(0, 0));   // conj(C_i) C_j
+                            zre = fma(z.re, x, zre);
+                            zim = fma(z.im, x, zim);
+                        }
+                        r0 += ng;
+                    }
+                    if (J.ok) {
+                        atomicAdd(&acc[J.slot].re, zre);
+                        atomicAdd(&acc[J.slot].im, zim);
+                    }
+                }
+            }
+        }
+        __builtin_amdgcn_wave_barrier();   // the rows are rewritten for the next wavefront
+    }
+    __syncthreads();
     for (int x = t; x < nterms; x += T) partial[(int64_t)blockIdx.x * nterms + x] = acc[x];
 }
 
@@ -374,12 +548,19 @@ __global__ void k_sum_blocks(int nblocks, int nterms, const cplx *__restrict__ p
 }
 
 // run one class and bring its compact terms to the host
-template <int CLASS, int R>
-static void launch_rho(const BasisDev &b, const cplx *C, int p0, int p1, int nterms, cplx *partial, int nblocks,
-                       int T, size_t sm)
+template <bool ALL, int R>
+static void launch_rho(const BasisDev &b, const cplx *C, int nterms, cplx *partial, int nblocks, int T, size_t sm)
 {
-    (void)hipFuncSetAttribute((const void *)k_rho<CLASS, R>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)sm);
-    hipLaunchKernelGGL((k_rho<CLASS, R>), dim3(nblocks), dim3(T), sm, 0, b, C, p0, p1, nterms, partial);
+    (void)hipFuncSetAttribute((const void *)k_rho<ALL, R>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)sm);
+    hipLaunchKernelGGL((k_rho<ALL, R>), dim3(nblocks), dim3(T), sm, 0, b, C, nterms, partial);
+}
+
+template <int CLASS>
+static void launch_jumps(const BasisDev &b, const cplx *C, int p0, int p1, int nterms, cplx *partial, int nblocks,
+                         int T, size_t sm)
+{
+    (void)hipFuncSetAttribute((const void *)k_rho_jumps<CLASS>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)sm);
+    hipLaunchKernelGGL((k_rho_jumps<CLASS>), dim3(nblocks), dim3(T), sm, 0, b, C, p0, p1, nterms, partial);
 }
 
 static mcpi_status run_class(mcpi_basis bs, int cls, const cplx *C, int p0, int p1, int nterms,
@@ -392,9 +573,30 @@ static mcpi_status run_class(mcpi_basis bs, int cls, const cplx *C, int p0, int 
     MCPI_HIP(hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev));
     const size_t max_sm = 160 * 1024;
     const size_t base_sm = sizeof(cplx) * nterms + sizeof(double) * (b.N + 3) + sizeof(int32_t) * b.M * (b.N + 1) + 16;
-    // rows per lane: as many as the occupation tile leaves room for, at most
-    // 4 for class 0 and 2 for the batched double-jump classes (4 spills)
-    int R = cls == 0 ? 4 : 2;
+    if (cls == 1 || cls == 2) {
+        // lane per term: 16 wavefronts a block, one LDS row tile (occupations
+        // and prefix counts) each
+        const size_t sm = base_sm + 2 * (size_t)T * ((b.M + 3) & ~3);
+        MCPI_ARG(sm <= max_sm, "density-matrix accumulators exceed LDS");
+        const int64_t nwv = (b.nc + 63) / 64;
+        const int nblocks = (int)std::min<int64_t>(std::max<int64_t>((nwv + T / 64 - 1) / (T / 64), 1), (int64_t)ncu);
+        cplx *partial = nullptr, *out = nullptr;
+        MCPI_HIP(hipMalloc(&partial, sizeof(cplx) * (size_t)nblocks * nterms));
+        MCPI_HIP(hipMalloc(&out, sizeof(cplx) * nterms));
+        if (cls == 1) launch_jumps<1>(b, C, p0, p1, nterms, partial, nblocks, T, sm);
+        else launch_jumps<2>(b, C, p0, p1, nterms, partial, nblocks, T, sm);
+        MCPI_HIP(hipGetLastError());
+        hipLaunchKernelGGL(k_sum_blocks, dim3((nterms + 255) / 256), dim3(256), 0, 0, nblocks, nterms, partial, out);
+        MCPI_HIP(hipGetLastError());
+        host.resize(nterms);
+        MCPI_HIP(hipMemcpy(host.data(), out, sizeof(cplx) * nterms, hipMemcpyDeviceToHost));
+        MCPI_HIP(hipFree(partial));
+        MCPI_HIP(hipFree(out));
+        return MCPI_OK;
+    }
+    // classes 0 and 3, rows per lane: as many as the occupation tile leaves
+    // room for, at most 2 (class 0; 4 spills) or 4 (class 3)
+    int R = cls == 0 ? 2 : 4;
     while (R > 1 && base_sm + (size_t)b.M * T * R > max_sm) R /= 2;
     // and few enough that every CU gets a tile
     while (R > 1 && (b.nc + (int64_t)T * R - 1) / ((int64_t)T * R) < ncu) R /= 2;
@@ -406,15 +608,12 @@ static mcpi_status run_class(mcpi_basis bs, int cls, const cplx *C, int p0, int 
     MCPI_HIP(hipMalloc(&partial, sizeof(cplx) * (size_t)nblocks * nterms));
     MCPI_HIP(hipMalloc(&out, sizeof(cplx) * nterms));
     if (cls == 0) {
-        if (R == 4) launch_rho<0, 4>(b, C, p0, p1, nterms, partial, nblocks, T, sm);
-        else if (R == 2) launch_rho<0, 2>(b, C, p0, p1, nterms, partial, nblocks, T, sm);
-        else launch_rho<0, 1>(b, C, p0, p1, nterms, partial, nblocks, T, sm);
-    } else if (cls == 1) {
-        if (R == 2) launch_rho<1, 2>(b, C, p0, p1, nterms, partial, nblocks, T, sm);
-        else launch_rho<1, 1>(b, C, p0, p1, nterms, partial, nblocks, T, sm);
+        if (R == 2) launch_rho<true, 2>(b, C, nterms, partial, nblocks, T, sm);
+        else launch_rho<true, 1>(b, C, nterms, partial, nblocks, T, sm);
     } else {
-        if (R == 2) launch_rho<2, 2>(b, C, p0, p1, nterms, partial, nblocks, T, sm);
-        else launch_rho<2, 1>(b, C, p0, p1, nterms, partial, nblocks, T, sm);
+        if (R == 4) launch_rho<false, 4>(b, C, nterms, partial, nblocks, T, sm);
+        else if (R == 2) launch_rho<false, 2>(b, C, nterms, partial, nblocks, T, sm);
+        else launch_rho<false, 1>(b, C, nterms, partial, nblocks, T, sm);
     }
     MCPI_HIP(hipGetLastError());
     hipLaunchKernelGGL(k_sum_blocks, dim3((nterms + 255) / 256), dim3(256), 0, 0, nblocks, nterms, partial, out);
@@ -439,7 +638,9 @@ static mcpi_status densities(mcpi_basis bs, const cplx *C, cplx *rho1, cplx *rho
     const int P = M * (M - 1) / 2;
     std::vector<cplx> a;
     const int nA = M + P + M + P + P + P + P * M;
-    MCPI_CHECK(run_class(bs, 0, C, 0, 0, nA, a));
+    // class 0 in full for the two-body density, else its r1d and r1o terms
+    // only (class 3)
+    MCPI_CHECK(run_class(bs, rho2 ? 0 : 3, C, 0, 0, nA, a));
     const cplx *r1d = a.data(), *r1o = r1d + M, *R1 = r1o + P, *R2 = R1 + M, *R4 = R2 + P,
                *R5 = R4 + P, *R7 = R5 + P;
     if (rho1) {

@@ -12,11 +12,14 @@ from tests.gpu_util import assert_close, note
 pytestmark = pytest.mark.gpu
 
 
-@pytest.mark.parametrize("N,M", [(1, 3), (2, 3), (5, 5), (4, 6), (6, 4), (3, 7), (7, 9)])
+@pytest.mark.parametrize("N,M", [(1, 3), (2, 3), (5, 5), (4, 6), (6, 4), (3, 7), (7, 9), (3, 12), (2, 14), (2, 17)])
 @pytest.mark.parametrize("maps", ["full", "single", "none"])
 def test_rho_matches_oracle(oracle_lib, N, M, maps):
     """Targets read from the stored maps (full), or ranked on the fly for the
-    double jumps (single) or for every jump (none)."""
+    double jumps (single) or for every jump (none).  M = 12 and 14: same-orbital
+    jumps with more than 64 terms (two passes of the lanes); M = 14, 17: the
+    two-orbital accumulators split over pair ranges; M = 17: ranked targets
+    per term instead of per-row difference tables (M > 16)."""
     note(f"rho N={N} M={M} maps={maps}")
     o = oracle.Basis(N, M)
     C, _, _ = oracle.random_inputs(N, M, seed=21 + N)
@@ -35,9 +38,11 @@ def test_rho_golden():
     assert_close(b.rho2(dC), gd["rho2"], 1e-12, "rho2")
 
 
-def test_reference_shaped_rho(oracle_lib):
-    N, M = 5, 5
-    note("OBrho_XM / TBrho_XX")
+@pytest.mark.parametrize("N,M", [(5, 5), (2, 17)])
+def test_reference_shaped_rho(oracle_lib, N, M):
+    """Reference-layout maps passed in; M = 17 reads the double-jump targets
+    from them."""
+    note(f"OBrho_XM / TBrho_XX N={N} M={M}")
     r = R.RefBasis(N, M)
     o = oracle.Basis(N, M)
     C, _, _ = oracle.random_inputs(N, M, seed=1)
