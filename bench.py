@@ -329,6 +329,21 @@ def main() -> int:
                 others[label]["pattern_table_bytes"] = b.pattern_bytes()
             agree = max(agree, float(np.linalg.norm(dO.numpy() - gpu_out) / np.linalg.norm(gpu_out)))
 
+    # one- and two-body densities of the same vector on the same basis
+    # (calls include the device-to-host copy of the representative terms
+    # and the host's symmetry expansion)
+    dens = None
+    if not args.profile:
+        b.rho1(dC), b.rho2(dC)
+        mcpi_amd.synchronize()
+        dens = {}
+        for label, fn in (("rho1_ms", b.rho1), ("rho2_ms", b.rho2)):
+            t = time.perf_counter()
+            for _ in range(5):
+                fn(dC)
+            mcpi_amd.synchronize()
+            dens[label] = round((time.perf_counter() - t) / 5 * 1e3, 3)
+
     batched = large = None
     if not (args.no_batched or args.profile):
         del dO
@@ -377,6 +392,7 @@ def main() -> int:
             "kernel_ms_total": round(float(each.sum()), 3),
             "basis_build_s": round(build_s, 3),
             "other_schedules": others,
+            "densities_n12_m12": dens,
             "batched_lanczos": batched,
             "large_basis_n16_m12": large,
             "max_rel_diff_between_schedules": agree,

@@ -99,3 +99,27 @@ def test_host_tridiagonal_solver_matches_numpy():
     e = np.full(d.size - 1, 1e-3)
     w = np.linalg.eigvalsh(np.diag(d) + np.diag(e, 1) + np.diag(e, -1))
     assert abs(mcpi_amd.tridiag_lowest(d, e)[0] - w[0]) < 1e-12 * 50
+
+
+def test_tridiagonal_solvers_match_lapack_dstev(oracle_lib):
+    """The Ritz step's tridiagonal solve (groundStateLieb.c:435 calls LAPACK
+    dstev) against dstev itself, here from SciPy's LAPACK: lowest eigenvalue
+    and its eigenvector (up to sign), for the library's host solver and the
+    oracle's, on random and Lanczos-shaped tridiagonals."""
+    from scipy.linalg.lapack import dstev
+    rng = np.random.default_rng(7)
+    cases = []
+    for n in (1, 2, 3, 17, 120, 200):
+        cases.append((rng.standard_normal(n) * 5, rng.standard_normal(max(n - 1, 0))))
+    # Lanczos-shaped: decaying off-diagonal, well-separated bottom, growing diagonal
+    n = 150
+    cases.append((np.linspace(-30.0, 400.0, n) + rng.standard_normal(n), 10.0 * np.exp(-np.arange(n - 1) / 40.0)))
+    for d, e in cases:
+        w, V, info = dstev(d, e if e.size else np.zeros(1), compute_v=1)   # e: max(1, n - 1) entries
+        assert info == 0
+        gap = (w[1] - w[0]) if d.size > 1 else 1.0
+        scale = max(1.0, float(np.abs(w).max()))
+        for lam, z in (mcpi_amd.tridiag_lowest(d, e), oracle.tridiag_min(d, e)):
+            assert abs(lam - w[0]) <= 1e-12 * scale
+            z = np.asarray(z) * np.sign(np.dot(np.asarray(z), V[:, 0]))
+            assert np.linalg.norm(z - V[:, 0]) <= 1e-12 * scale / gap + 1e-13
