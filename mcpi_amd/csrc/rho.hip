@@ -567,16 +567,19 @@ static mcpi_status run_class(mcpi_basis bs, int cls, const cplx *C, int p0, int 
                              std::vector<cplx> &host)
 {
     const BasisDev &b = bs->d;
-    const int T = 1024;
+    int T = 1024;
     int dev = 0, ncu = 0;
     MCPI_HIP(hipGetDevice(&dev));
     MCPI_HIP(hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev));
     const size_t max_sm = 160 * 1024;
     const size_t base_sm = sizeof(cplx) * nterms + sizeof(double) * (b.N + 3) + sizeof(int32_t) * b.M * (b.N + 1) + 16;
     if (cls == 1 || cls == 2) {
-        // lane per term: 16 wavefronts a block, one LDS row tile (occupations
-        // and prefix counts) each
-        const size_t sm = base_sm + 2 * (size_t)T * ((b.M + 3) & ~3);
+        // lane per term: up to 16 wavefronts a block, one LDS row tile
+        // (occupations and prefix counts) each; fewer when the accumulators
+        // leave too little LDS (large M)
+        const size_t ms = (size_t)((b.M + 3) & ~3);
+        while (T > 64 && base_sm + 2 * (size_t)T * ms > max_sm) T /= 2;
+        const size_t sm = base_sm + 2 * (size_t)T * ms;
         MCPI_ARG(sm <= max_sm, "density-matrix accumulators exceed LDS");
         const int64_t nwv = (b.nc + 63) / 64;
         const int nblocks = (int)std::min<int64_t>(std::max<int64_t>((nwv + T / 64 - 1) / (T / 64), 1), (int64_t)ncu);

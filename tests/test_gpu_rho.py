@@ -12,14 +12,17 @@ from tests.gpu_util import assert_close, note
 pytestmark = pytest.mark.gpu
 
 
-@pytest.mark.parametrize("N,M", [(1, 3), (2, 3), (5, 5), (4, 6), (6, 4), (3, 7), (7, 9), (3, 12), (2, 14), (2, 17)])
+@pytest.mark.parametrize("N,M", [(1, 3), (2, 3), (5, 5), (4, 6), (6, 4), (3, 7), (7, 9), (3, 12), (2, 14), (2, 17),
+                                 (2, 24)])
 @pytest.mark.parametrize("maps", ["full", "single", "none"])
 def test_rho_matches_oracle(oracle_lib, N, M, maps):
     """Targets read from the stored maps (full), or ranked on the fly for the
     double jumps (single) or for every jump (none).  M = 12 and 14: same-orbital
     jumps with more than 64 terms (two passes of the lanes); M = 14, 17: the
-    two-orbital accumulators split over pair ranges; M = 17: ranked targets
-    per term instead of per-row difference tables (M > 16)."""
+    two-orbital accumulators split over pair ranges; M = 17, 24: ranked
+    targets per term instead of per-row difference tables (M > 16); M = 24:
+    the largest M whose class-0 accumulators fit in LDS, double-jump blocks
+    narrowed to leave room for them."""
     note(f"rho N={N} M={M} maps={maps}")
     o = oracle.Basis(N, M)
     C, _, _ = oracle.random_inputs(N, M, seed=21 + N)
