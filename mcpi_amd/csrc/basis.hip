@@ -514,6 +514,7 @@ static void free_basis(mcpi_basis b)
         for (void *p : q)
             if (p) (void)hipFree(p);
     }
+    if (b->rho_ws) (void)hipFree(b->rho_ws);
     delete b;
 }
 

@@ -3,6 +3,7 @@
 
 #include <hip/hip_runtime.h>
 
+#include <mutex>
 #include <vector>
 #include <stdint.h>
 
@@ -156,6 +157,11 @@ struct mcpi_basis_s {
     int64_t device_bytes;
     std::vector<mcpi::PatternTables> patterns;   // built on demand, a few zero patterns
     uint64_t use_clock = 0;
+    // density-matrix scratch (block partials and their sums), grown on
+    // demand and kept; one density call at a time per basis uses it
+    void *rho_ws = nullptr;
+    size_t rho_ws_bytes = 0;
+    std::mutex rho_mu;
 };
 
 // Folded coefficient tables of a Hamiltonian (device memory), produced on

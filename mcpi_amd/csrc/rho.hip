@@ -29,6 +29,7 @@
 
 #include <cmath>
 #include <cstring>
+#include <mutex>
 #include <vector>
 
 #include "mcpi_internal.h"
@@ -232,7 +233,8 @@ __device__ inline int64_t readlane_l(int64_t x, int lane)
 // number of reductions and LDS atomics by R.  ALL: every class-0 term (the
 // two-body density); else r1d and r1o only (the one-body density).  With
 // ALL the terms of a pair (k, l) (R2, r1o, R4, R5 and the M - 2 terms of
-// R7) are reduced 8 at a time (wave_add8).
+// R7) are reduced 8 at a time (wave_add8); without, the r1o terms of eight
+// pairs.
 template <bool ALL, int R>
 __global__ __launch_bounds__(1024) void k_rho(BasisDev b, const cplx *__restrict__ C, int nterms,
                                              cplx *__restrict__ partial)
@@ -281,8 +283,41 @@ __global__ __launch_bounds__(1024) void k_rho(BasisDev b, const cplx *__restrict
                 if (ALL) wave_add(cmake(a2, 0), acc, M + P + k, wave_any(any2));
             }
             const int o_r1o = M, o_R2 = M + P + M, o_R4 = o_R2 + P, o_R5 = o_R4 + P, o_R7 = o_R5 + P;
+            if (!ALL) {
+                // r1o only: eight pairs a batch, their targets and gathers
+                // in flight together and one batched reduction
+                int k = 0, l = 1;
+                for (int pq0 = 0; pq0 < P; pq0 += 8) {
+                    cplx zb[8];
+                    int sb[8];
+                    uint32_t use = 0;
+#pragma unroll
+                    for (int u = 0; u < 8; u++) {
+                        zb[u] = cmake(0, 0);
+                        sb[u] = 0;
+                        if (pq0 + u >= P) continue;   // uniform
+                        use |= 1u << u;
+                        sb[u] = o_r1o + pq0 + u;
+#pragma unroll
+                        for (int r = 0; r < R; r++) {
+                            const int vk = V(k, r), vl = V(l, r);
+                            if (valid[r] && vk >= 1) {
+                                const int64_t j = b.Map ? (int64_t)b.Map[i[r] + (int64_t)(k + M * l) * nc]
+                                                        : rank_lane(Vs, t + T * r, TR, Bt, N, M, nc, k, -1, l, -1);
+                                const cplx g = cfma_conj(ci[r], ldcr(C + j), cmake(0, 0));   // conj(C_i) C_j
+                                zb[u] = cfma(g, cmake(sq[vk] * sq[vl + 1], 0), zb[u]);
+                            }
+                        }
+                        if (++l == M) {
+                            ++k;
+                            l = k + 1;
+                        }
+                    }
+                    wave_add8(zb, acc, sb, use);
+                }
+            }
             int pq = 0;
-            for (int k = 0; k < M; k++)
+            for (int k = 0; ALL && k < M; k++)
                 for (int l = k + 1; l < M; l++, pq++) {
                     // R2: |C|^2 v_k v_l over all rows
                     double a2 = 0;
@@ -309,14 +344,8 @@ __global__ __launch_bounds__(1024) void k_rho(BasisDev b, const cplx *__restrict
                         }
                         root[r] = act ? sq[vk] * sq[vl + 1] : 0.0;
                         z1 = cfma(g[r], cmake(root[r], 0), z1);
-                        if (ALL) {
-                            z4 = cfma(g[r], cmake(act && vk >= 2 ? (vk - 1) * root[r] : 0.0, 0), z4);
-                            z5 = cfma(g[r], cmake(act && vl >= 1 ? vl * root[r] : 0.0, 0), z5);
-                        }
-                    }
-                    if (!ALL) {
-                        wave_add(z1, acc, o_r1o + pq, true);
-                        continue;
+                        z4 = cfma(g[r], cmake(act && vk >= 2 ? (vk - 1) * root[r] : 0.0, 0), z4);
+                        z5 = cfma(g[r], cmake(act && vl >= 1 ? vl * root[r] : 0.0, 0), z5);
                     }
                     // terms x of the pair: R2, r1o, R4, R5, then R7 for the
                     // x - 4'th orbital s other than k and l
@@ -555,6 +584,23 @@ static void launch_rho(const BasisDev &b, const cplx *C, int nterms, cplx *parti
     hipLaunchKernelGGL((k_rho<ALL, R>), dim3(nblocks), dim3(T), sm, 0, b, C, nterms, partial);
 }
 
+// the basis's density scratch: nblocks * nterms block partials, then
+// nterms sums (grown on demand; the caller holds rho_mu)
+static mcpi_status density_scratch(mcpi_basis bs, int nblocks, int nterms, cplx **partial, cplx **out)
+{
+    const size_t need = sizeof(cplx) * ((size_t)nblocks * nterms + (size_t)nterms);
+    if (bs->rho_ws_bytes < need) {
+        if (bs->rho_ws) MCPI_HIP(hipFree(bs->rho_ws));
+        bs->rho_ws = nullptr;
+        bs->rho_ws_bytes = 0;
+        MCPI_HIP(hipMalloc(&bs->rho_ws, need));
+        bs->rho_ws_bytes = need;
+    }
+    *partial = (cplx *)bs->rho_ws;
+    *out = *partial + (size_t)nblocks * nterms;
+    return MCPI_OK;
+}
+
 template <int CLASS>
 static void launch_jumps(const BasisDev &b, const cplx *C, int p0, int p1, int nterms, cplx *partial, int nblocks,
                          int T, size_t sm)
@@ -584,8 +630,7 @@ static mcpi_status run_class(mcpi_basis bs, int cls, const cplx *C, int p0, int 
         const int64_t nwv = (b.nc + 63) / 64;
         const int nblocks = (int)std::min<int64_t>(std::max<int64_t>((nwv + T / 64 - 1) / (T / 64), 1), (int64_t)ncu);
         cplx *partial = nullptr, *out = nullptr;
-        MCPI_HIP(hipMalloc(&partial, sizeof(cplx) * (size_t)nblocks * nterms));
-        MCPI_HIP(hipMalloc(&out, sizeof(cplx) * nterms));
+        MCPI_CHECK(density_scratch(bs, nblocks, nterms, &partial, &out));
         if (cls == 1) launch_jumps<1>(b, C, p0, p1, nterms, partial, nblocks, T, sm);
         else launch_jumps<2>(b, C, p0, p1, nterms, partial, nblocks, T, sm);
         MCPI_HIP(hipGetLastError());
@@ -593,8 +638,6 @@ static mcpi_status run_class(mcpi_basis bs, int cls, const cplx *C, int p0, int 
         MCPI_HIP(hipGetLastError());
         host.resize(nterms);
         MCPI_HIP(hipMemcpy(host.data(), out, sizeof(cplx) * nterms, hipMemcpyDeviceToHost));
-        MCPI_HIP(hipFree(partial));
-        MCPI_HIP(hipFree(out));
         return MCPI_OK;
     }
     // classes 0 and 3, rows per lane: as many as the occupation tile leaves
@@ -608,8 +651,7 @@ static mcpi_status run_class(mcpi_basis bs, int cls, const cplx *C, int p0, int 
     int64_t tiles = (b.nc + (int64_t)T * R - 1) / ((int64_t)T * R);
     int nblocks = (int)std::min<int64_t>(std::max<int64_t>(tiles, 1), (int64_t)ncu);
     cplx *partial = nullptr, *out = nullptr;
-    MCPI_HIP(hipMalloc(&partial, sizeof(cplx) * (size_t)nblocks * nterms));
-    MCPI_HIP(hipMalloc(&out, sizeof(cplx) * nterms));
+    MCPI_CHECK(density_scratch(bs, nblocks, nterms, &partial, &out));
     if (cls == 0) {
         if (R == 2) launch_rho<true, 2>(b, C, nterms, partial, nblocks, T, sm);
         else launch_rho<true, 1>(b, C, nterms, partial, nblocks, T, sm);
@@ -623,8 +665,6 @@ static mcpi_status run_class(mcpi_basis bs, int cls, const cplx *C, int p0, int 
     MCPI_HIP(hipGetLastError());
     host.resize(nterms);
     MCPI_HIP(hipMemcpy(host.data(), out, sizeof(cplx) * nterms, hipMemcpyDeviceToHost));
-    MCPI_HIP(hipFree(partial));
-    MCPI_HIP(hipFree(out));
     return MCPI_OK;
 }
 
@@ -637,6 +677,7 @@ static inline cplx conjc(cplx z) { return cmake(z.re, -z.im); }
 // rho1 / rho2 from the device; either output may be null
 static mcpi_status densities(mcpi_basis bs, const cplx *C, cplx *rho1, cplx *rho2)
 {
+    std::lock_guard<std::mutex> lock(bs->rho_mu);
     const int M = bs->d.M;
     const int P = M * (M - 1) / 2;
     std::vector<cplx> a;
