@@ -54,10 +54,13 @@ def traffic_from_profiles():
     return None
 
 
-def cpu_baseline(b, C, Ho, Hint, gpu_out, target_s: float = 12.0):
+def cpu_baseline(b, C, Ho, Hint, gpu_out, target_s: float = 12.0, gpu_rho=None):
     """Time the oracle's OpenMP H.v (reference applyHconf_omp restated, full
     maps with int64 offsets) over the whole basis on the host cores, repeated
-    until about target_s seconds have run; check each result against the GPU."""
+    until about target_s seconds have run; check each result against the GPU.
+    With gpu_rho = (rho1, rho2) from the GPU, also time the oracle's density
+    matrices (OBrho_XM / TBrho_XX restated) once each on the same tables and
+    check them."""
     import oracle
     L = oracle.lib()
     nc, M = b.nc, b.M
@@ -80,6 +83,20 @@ def cpu_baseline(b, C, Ho, Hint, gpu_out, target_s: float = 12.0):
         rel = max(rel, float(np.linalg.norm(out - gpu_out) / np.linalg.norm(gpu_out)))
     t_hv = float(np.median(times))
     B = algorithmic_bytes(nc, M, b.ot_len, b.tt_len)
+    dens = None
+    if gpu_rho is not None:
+        r1 = np.zeros(M * M, np.complex128)
+        r2 = np.zeros(M ** 4, np.complex128)
+        t = time.perf_counter()
+        L.orc_rho1(N_, M, Map, IF, C, r1)
+        t1 = time.perf_counter() - t
+        t = time.perf_counter()
+        L.orc_rho2(N_, M, Map, MapOT, MapTT, sot, stt, IF, C, r2)
+        t2 = time.perf_counter() - t
+        d1 = float(np.abs(oracle.unflat(r1, M, 2) - gpu_rho[0]).max() / np.abs(gpu_rho[0]).max())
+        d2 = float(np.abs(oracle.unflat(r2, M, 4) - gpu_rho[1]).max() / np.abs(gpu_rho[1]).max())
+        dens = {"rho1_ms": round(t1 * 1e3, 1), "rho2_ms": round(t2 * 1e3, 1), "cores": threads,
+                "max_rel_diff_vs_gpu": max(d1, d2)}
     return {
         "value": round(B / t_hv / 1e9, 3),
         "unit": "GB/s",
@@ -90,6 +107,7 @@ def cpu_baseline(b, C, Ho, Hint, gpu_out, target_s: float = 12.0):
                    f"per H.v; results match the GPU to {rel:.1e} (relative 2-norm)"),
         "ms_per_hv": round(t_hv * 1e3, 2),
         "table_copy_s": round(copy_s, 2),
+        "densities": dens,
     }
 
 
@@ -332,9 +350,9 @@ def main() -> int:
     # one- and two-body densities of the same vector on the same basis
     # (calls include the device-to-host copy of the representative terms
     # and the host's symmetry expansion)
-    dens = None
+    dens = gpu_rho = None
     if not args.profile:
-        b.rho1(dC), b.rho2(dC)
+        gpu_rho = (b.rho1(dC), b.rho2(dC))
         mcpi_amd.synchronize()
         dens = {}
         for label, fn in (("rho1_ms", b.rho1), ("rho2_ms", b.rho2)):
@@ -351,7 +369,7 @@ def main() -> int:
 
     cpu = None
     if not (args.no_cpu or args.profile):
-        cpu = cpu_baseline(b, C, Ho, Hint, gpu_out, args.cpu_seconds)
+        cpu = cpu_baseline(b, C, Ho, Hint, gpu_out, args.cpu_seconds, gpu_rho)
     if not (args.no_large or args.profile):
         large = large_basis()
 
