@@ -12,7 +12,10 @@
 // Layout as for the batched kernel: instance v's coefficient block is
 // instance 0's shifted by v * cstride bytes, its vectors sit v * vstride
 // elements after C0 / out0.  The 1-D grid holds ceil(nbatch / NV) blocks per
-// row block, group index fastest after the XCD remap.
+// row block, group index fastest; chunks of 32 row blocks go to the XCDs
+// round-robin (chunk_xcd), so a row block's groups share an L2 (batched
+// Lanczos, 64 instances: 1.31 s per call, 1.35 s with each XCD walking one
+// contiguous eighth of the rows).
 #pragma once
 
 #include "acth_fast.h"
@@ -108,7 +111,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void k
     int64_t vstride)
 {
     const unsigned G = (unsigned)((nbatch + NV - 1) / NV);
-    const unsigned vb = xcd_block(blockIdx.x, gridDim.x);
+    const unsigned vb = chunk_xcd(blockIdx.x, gridDim.x, 32 * G);
     const unsigned grp = vb % G, rb = vb / G;
     const int vfirst = (int)grp * NV;
     const int nv = min(NV, nbatch - vfirst);
