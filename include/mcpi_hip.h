@@ -90,13 +90,29 @@ void applyHconf_omp(int N, int M, Iarray Map, Iarray MapOT, Iarray MapTT,
 void applyHconf_XX(int N, int M, Iarray Map, Iarray MapOT, Iarray MapTT,
                    Iarray strideOT, Iarray strideTT, Iarray IF, Carray C,
                    Cmatrix Ho, Carray Hint, Carray out);
+/* hamiltonianMatrix.h:431  same computation, Map and MapOT given (double
+ * jumps are ranked on the fly; MapOT holds the same targets) */
+void applyHconf_X(int N, int M, Iarray Map, Iarray MapOT, Iarray strideOT,
+                  Iarray NCmat, Iarray IF, Carray C, Cmatrix Ho, Carray Hint,
+                  Carray out);
 /* hamiltonianMatrix.h:31  hash-only H.v (double jumps ranked on the fly) */
 void applyHconf(int N, int M, Iarray NCmat, Iarray IF, Carray C, Cmatrix Ho,
                 Carray Hint, Carray out);
-/* onebodyMatrix.h:190  rho[k][l] = <a*_k a_l> */
+/* onebodyMatrix.h:34  rho[k][l] = <a*_k a_l>, no table passed (the basis is
+ * generated on the GPU from N, M) */
+void OBrho(int N, int M, Iarray NCmat, Carray C, Cmatrix rho);
+/* onebodyMatrix.h:116  same, IF given (the reference edits IF in place and
+ * restores it; this one only reads it) */
+void OBrho_X(int N, int M, Iarray NCmat, Iarray IF, Carray C, Cmatrix rho);
+/* onebodyMatrix.h:190  same, IF and Map given */
 void OBrho_XM(int N, int M, Iarray Map, Iarray NCmat, Iarray IF, Carray C,
               Cmatrix rho);
-/* twobodyMatrix.h:991  rho[k + sM + qM^2 + lM^3] = <a*_k a*_s a_q a_l> */
+/* twobodyMatrix.h:38  rho[k + sM + qM^2 + lM^3] = <a*_k a*_s a_q a_l>, IF given */
+void TBrho(int N, int M, Iarray NCmat, Iarray IF, Carray C, Carray rho);
+/* twobodyMatrix.h:502  same, IF, Map and MapOT given (double jumps ranked) */
+void TBrho_X(int N, int M, Iarray Map, Iarray MapOT, Iarray strideOT,
+             Iarray NCmat, Iarray IF, Carray C, Carray rho);
+/* twobodyMatrix.h:991  same, every map given */
 void TBrho_XX(int N, int M, Iarray Map, Iarray MapOT, Iarray MapTT,
               Iarray strideOT, Iarray strideTT, Iarray NCmat, Iarray IF,
               Carray C, Carray rho);
@@ -159,6 +175,13 @@ typedef struct {
 
 const char *mcpi_hip_last_error(void);
 const char *mcpi_hip_version(void);
+/* The reference-shaped entry points keep device state between calls: the
+ * tables uploaded from the caller's arrays (reused while the same arrays are
+ * passed again, which are taken to be unchanged), the Hamiltonian of the
+ * last Ho/Hint with its tuned H.v schedule (reused while Ho and Hint are
+ * equal entry for entry) and two device vectors.  This frees all of it, e.g.
+ * after the caller edited tables in place or before it ends. */
+mcpi_status mcpi_hip_release_cached(void);
 
 /* device memory and synchronization helpers (HIP runtime of the library) */
 mcpi_status mcpi_hip_malloc(void **dptr, size_t bytes);
@@ -230,13 +253,24 @@ mcpi_status mcpi_hip_ham_get_schedule(mcpi_ham h, int *schedule);
 mcpi_status mcpi_hip_applyH(mcpi_ham h, const mcpi_cplx *C_dev, mcpi_cplx *out_dev,
                             void *stream);
 /* CUDA-kernel-shaped H.v on raw device tables (cuda/hamiltonianMatrix.cuh:61):
- * every pointer is device memory, Ho flat k + M*l, int64 strides. */
+ * every pointer is device memory, Ho flat k + M*l, int64 strides (nc
+ * entries).  Every term is evaluated.  Enqueued on stream (NULL = default)
+ * after a stream-ordered read of Ho and Hint; the folded coefficients of the
+ * last Ho/Hint and the rank table of the last (N, M) are kept and reused. */
 mcpi_status mcpi_hip_applyHconf_dev(int N, int M, const int32_t *Map,
                                     const int32_t *MapOT, const int32_t *MapTT,
                                     const int64_t *strideOT, const int64_t *strideTT,
                                     const int32_t *IF, const mcpi_cplx *C,
                                     const mcpi_cplx *Ho, const mcpi_cplx *Hint,
                                     mcpi_cplx *out, void *stream);
+/* the same with the CUDA kernel's int32 strides (Iarray strideOT/strideTT,
+ * cuda/hamiltonianMatrix.cuh:61-63), widened on the device when read */
+mcpi_status mcpi_hip_applyHconf_dev32(int N, int M, const int32_t *Map,
+                                      const int32_t *MapOT, const int32_t *MapTT,
+                                      const int32_t *strideOT, const int32_t *strideTT,
+                                      const int32_t *IF, const mcpi_cplx *C,
+                                      const mcpi_cplx *Ho, const mcpi_cplx *Hint,
+                                      mcpi_cplx *out, void *stream);
 /* average duration (ms) of the H.v kernels of the last n calls of
  * mcpi_hip_applyH_timed, measured with HIP events on the launch stream */
 mcpi_status mcpi_hip_applyH_timed(mcpi_ham h, const mcpi_cplx *C_dev,

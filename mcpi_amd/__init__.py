@@ -23,8 +23,10 @@ Two layers are exposed:
 """
 from __future__ import annotations
 
+import atexit
 import ctypes as ct
 import os
+import weakref
 
 import numpy as np
 
@@ -53,6 +55,10 @@ class _BasisInfo(ct.Structure):
 
 
 _lib = None
+# Live handles, released in a fixed order when the interpreter exits (see
+# _shutdown); afterwards the handles' __del__ do nothing.
+_live: "weakref.WeakSet" = weakref.WeakSet()
+_finalized = False
 
 
 def lib():
@@ -60,6 +66,8 @@ def lib():
     global _lib
     if _lib is not None:
         return _lib
+    if _finalized:
+        raise MCPIError("mcpi_amd has been shut down (interpreter exit)")
     if not os.path.exists(LIB_PATH):
         raise MCPIError(f"{LIB_PATH} is missing: build it with `python -m mcpi_amd.build` "
                         "(the HIP library is required; there is no CPU fallback)")
@@ -96,6 +104,8 @@ def lib():
         "mcpi_hip_applyH": (st, [_vp, _vp, _vp, _vp]),
         "mcpi_hip_applyH_timed": (st, [_vp, _vp, _vp, ct.c_int, _vp, ct.POINTER(ct.c_double)]),
         "mcpi_hip_applyHconf_dev": (st, [ct.c_int, ct.c_int] + [_vp] * 11),
+        "mcpi_hip_applyHconf_dev32": (st, [ct.c_int, ct.c_int] + [_vp] * 11),
+        "mcpi_hip_release_cached": (st, []),
         "mcpi_hip_rho1": (st, [_vp, _vp, _c128p]),
         "mcpi_hip_rho2": (st, [_vp, _vp, _c128p]),
         "mcpi_hip_vjp": (st, [_vp, _vp, _c128p, _c128p]),
@@ -114,6 +124,32 @@ def lib():
         f.argtypes = args
     _lib = L
     return L
+
+
+def _shutdown() -> None:
+    """Release every live handle while the HIP runtime is still up: wait for
+    the device, then Hamiltonians, device arrays and bases in that order
+    (a Hamiltonian refers to its basis).  Runs first thing at interpreter
+    exit, before module teardown, so no __del__ reaches the library late."""
+    global _finalized
+    if _finalized:
+        return
+    try:
+        if _lib is not None:
+            _lib.mcpi_hip_synchronize()
+            _lib.mcpi_hip_release_cached()
+            for kind in (Hamiltonian, DeviceArray, Basis):
+                for obj in [o for o in list(_live) if isinstance(o, kind)]:
+                    try:
+                        obj.close()
+                    except Exception:
+                        pass
+            _lib.mcpi_hip_synchronize()
+    finally:
+        _finalized = True
+
+
+atexit.register(_shutdown)
 
 
 def _check(status: int, what: str = "") -> None:
@@ -144,6 +180,7 @@ class DeviceArray:
         p = _vp()
         _check(lib().mcpi_hip_malloc(ct.byref(p), max(self.nbytes, 16)), "malloc")
         self.ptr = p.value
+        _live.add(self)
 
     @classmethod
     def from_numpy(cls, a) -> "DeviceArray":
@@ -166,11 +203,15 @@ class DeviceArray:
         return self.ptr + offset_elems * self.dtype.itemsize
 
     def free(self) -> None:
-        if getattr(self, "ptr", None):
+        if getattr(self, "ptr", None) and not _finalized:
             lib().mcpi_hip_free(self.ptr)
-            self.ptr = None
+        self.ptr = None
+
+    close = free
 
     def __del__(self):
+        if _finalized:
+            return
         try:
             self.free()
         except Exception:
@@ -201,6 +242,7 @@ class Basis:
         h = _vp()
         _check(lib().mcpi_hip_basis_create(N, M, _MAPS[maps], ct.byref(h)), f"basis({N},{M})")
         self.handle = h.value
+        _live.add(self)
         info = _BasisInfo()
         _check(lib().mcpi_hip_basis_info(self.handle, ct.byref(info)), "basis_info")
         self.N, self.M, self.maps = info.N, info.M, info.maps
@@ -285,11 +327,13 @@ class Basis:
         return unflat(g1, self.M, 2), unflat(g2, self.M, 4)
 
     def close(self) -> None:
-        if getattr(self, "handle", None):
+        if getattr(self, "handle", None) and not _finalized:
             lib().mcpi_hip_basis_destroy(self.handle)
-            self.handle = None
+        self.handle = None
 
     def __del__(self):
+        if _finalized:
+            return
         try:
             self.close()
         except Exception:
@@ -305,6 +349,7 @@ class Hamiltonian:
         h = _vp()
         _check(lib().mcpi_hip_ham_create(basis.handle, flat(Ho), flat(Hint), ct.byref(h)), "ham_create")
         self.handle = h.value
+        _live.add(self)
         self.set_skip_zero(skip_zero)
         self.set_schedule(schedule)
 
@@ -369,11 +414,13 @@ class Hamiltonian:
         return E0.value, dO, steps.value
 
     def close(self) -> None:
-        if getattr(self, "handle", None):
+        if getattr(self, "handle", None) and not _finalized:
             lib().mcpi_hip_ham_destroy(self.handle)
-            self.handle = None
+        self.handle = None
 
     def __del__(self):
+        if _finalized:
+            return
         try:
             self.close()
         except Exception:
@@ -385,6 +432,13 @@ def applyHconf_dev(N, M, Map, MapOT, MapTT, strideOT, strideTT, IF, C, Ho, Hint,
     _check(lib().mcpi_hip_applyHconf_dev(N, M, Map.ptr, MapOT.ptr, MapTT.ptr, strideOT.ptr,
                                          strideTT.ptr, IF.ptr, C.ptr, Ho.ptr, Hint.ptr, out.ptr,
                                          None), "applyHconf_dev")
+
+
+def applyHconf_dev32(N, M, Map, MapOT, MapTT, strideOT, strideTT, IF, C, Ho, Hint, out) -> None:
+    """The same with int32 strides (the CUDA kernel's Iarray strides)."""
+    _check(lib().mcpi_hip_applyHconf_dev32(N, M, Map.ptr, MapOT.ptr, MapTT.ptr, strideOT.ptr,
+                                           strideTT.ptr, IF.ptr, C.ptr, Ho.ptr, Hint.ptr, out.ptr,
+                                           None), "applyHconf_dev32")
 
 
 def tridiag_lowest(d, e):

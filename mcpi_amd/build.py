@@ -2,7 +2,9 @@
 
     python -m mcpi_amd.build           # or mcpi_amd.build.build()
 
-Objects go to mcpi_amd/_build/, the library to mcpi_amd/libmcpi_hip.so.
+Objects go to mcpi_amd/_build/, the library to mcpi_amd/libmcpi_hip.so, and
+the Lieb-Liniger ground-state program (apps/groundStateLieb.c, linked against
+the library) to mcpi_amd/bin/groundStateLieb.
 Every source is compiled on every call (the build check must always
 compile); translation units are built in parallel.
 """
@@ -18,6 +20,8 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(HERE, "csrc")
 BUILD = os.path.join(HERE, "_build")
 LIB = os.path.join(HERE, "libmcpi_hip.so")
+BIN = os.path.join(HERE, "bin")
+APPS = ["groundStateLieb"]
 ARCH = os.environ.get("MCPI_OFFLOAD_ARCH", "gfx950")
 
 SOURCES = ["basis.hip", "acth.hip", "acth_fast_a.hip", "acth_fast_b.hip", "acth_fast_c.hip", "acth_comp.hip", "acth_multi_a.hip", "acth_multi_b.hip",
@@ -67,7 +71,23 @@ def build(verbose: bool = True) -> str:
     os.replace(tmp, LIB)
     if verbose:
         print(f"built {LIB}")
+    build_apps(verbose)
     return LIB
+
+
+def build_apps(verbose: bool = True) -> None:
+    """C programs of the package, linked against libmcpi_hip.so (rpath to it)."""
+    os.makedirs(BIN, exist_ok=True)
+    for app in APPS:
+        exe = os.path.join(BIN, app)
+        cmd = ["gcc", "-O2", "-std=c11", "-Wall", "-Werror", os.path.join(HERE, "apps", app + ".c"),
+               "-I", os.path.join(os.path.dirname(HERE), "include"), "-L", HERE, "-lmcpi_hip",
+               "-Wl,-rpath,$ORIGIN/..", "-lm", "-o", exe]
+        r = subprocess.run(cmd, capture_output=True, text=True)
+        if r.returncode != 0:
+            raise RuntimeError(f"build of {app} failed:\n{' '.join(cmd)}\n{r.stdout}\n{r.stderr}")
+        if verbose:
+            print(f"built {exe}")
 
 
 if __name__ == "__main__":

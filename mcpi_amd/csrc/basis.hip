@@ -789,6 +789,39 @@ mcpi_status basis_upload_raw(int N, int M, const int32_t *IF, const int32_t *NCm
     return MCPI_OK;
 }
 
+mcpi_status rank_tables_dev(int N, int M, int32_t **B, int32_t **NCmat)
+{
+    MCPI_ARG(N >= 0 && N <= MCPI_MAXN && M >= 1 && M <= MCPI_MAXM, "N or M out of range");
+    std::vector<int32_t> nch, bh;
+    host_tables(N, M, nch, bh);
+    *B = *NCmat = nullptr;
+    int64_t acct = 0;
+    MCPI_CHECK(alloc_dev((void **)B, bh.size() * 4 + 4, &acct));
+    mcpi_status st = alloc_dev((void **)NCmat, nch.size() * 4, &acct);
+    if (st == MCPI_OK && (hipMemcpy(*B, bh.data(), bh.size() * 4, hipMemcpyHostToDevice) != hipSuccess ||
+                          hipMemcpy(*NCmat, nch.data(), nch.size() * 4, hipMemcpyHostToDevice) != hipSuccess))
+        st = hip_fail(hipGetLastError(), "rank table upload", __FILE__, __LINE__);
+    if (st != MCPI_OK) {
+        if (*B) (void)hipFree(*B);
+        if (*NCmat) (void)hipFree(*NCmat);
+        *B = *NCmat = nullptr;
+    }
+    return st;
+}
+
+__global__ void k_widen32(int64_t n, const int32_t *__restrict__ a, int64_t *__restrict__ w)
+{
+    for (int64_t x = (int64_t)blockIdx.x * 256 + threadIdx.x; x < n; x += (int64_t)gridDim.x * 256) w[x] = a[x];
+}
+
+mcpi_status widen_strides(int64_t n, const int32_t *a, int64_t *w, hipStream_t st)
+{
+    const int64_t blocks = std::min<int64_t>(std::max<int64_t>((n + 255) / 256, 1), 4096);
+    hipLaunchKernelGGL(k_widen32, dim3((unsigned)blocks), dim3(256), 0, st, n, a, w);
+    MCPI_HIP(hipGetLastError());
+    return MCPI_OK;
+}
+
 mcpi_status basis_build_parts(mcpi_basis b, int want_map, int want_ot, int want_tt)
 {
     MCPI_ARG(b, "null basis");

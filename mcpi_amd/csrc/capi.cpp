@@ -38,13 +38,30 @@ mcpi_status lanczos_batched_api(int K, const mcpi_ham *hams, const cplx *const *
                                 int *steps, cplx *const *Cout, hipStream_t s);
 mcpi_status set_batched_kernel(int kind);
 
-// one library stream (default priority) for calls that do not pass one
-static hipStream_t lib_stream()
+// One library stream (default priority) for the Lanczos entry points, which
+// capture their iteration as a hipGraph (the legacy NULL stream cannot be
+// captured).  It does not synchronize with the NULL stream by itself, so
+// every call first makes it wait for the work already enqueued there (a
+// start vector just written by mcpi_hip_applyH(..., NULL), say).  The calls
+// that use it hold g_lz_mu: two threads must not enqueue onto it while one
+// of them captures.
+static std::mutex g_lz_mu;
+
+static mcpi_status lib_stream(hipStream_t *out)
 {
     static hipStream_t s = nullptr;
+    static hipEvent_t ev = nullptr;
     static std::once_flag once;
-    std::call_once(once, [] { (void)hipStreamCreateWithFlags(&s, hipStreamNonBlocking); });
-    return s;
+    static hipError_t init = hipSuccess;
+    std::call_once(once, [] {
+        init = hipStreamCreateWithFlags(&s, hipStreamNonBlocking);
+        if (init == hipSuccess) init = hipEventCreateWithFlags(&ev, hipEventDisableTiming);
+    });
+    if (init != hipSuccess) return hip_fail(init, "library stream", __FILE__, __LINE__);
+    MCPI_HIP(hipEventRecord(ev, nullptr));
+    MCPI_HIP(hipStreamWaitEvent(s, ev, 0));
+    *out = s;
+    return MCPI_OK;
 }
 
 }  // namespace mcpi
@@ -217,22 +234,64 @@ mcpi_status mcpi_hip_applyH_timed_each(mcpi_ham h, const mcpi_cplx *C_dev, mcpi_
     return st;
 }
 
-mcpi_status mcpi_hip_applyHconf_dev(int N, int M, const int32_t *Map, const int32_t *MapOT,
-                                    const int32_t *MapTT, const int64_t *strideOT, const int64_t *strideTT,
-                                    const int32_t *IF, const mcpi_cplx *C, const mcpi_cplx *Ho,
-                                    const mcpi_cplx *Hint, mcpi_cplx *out, void *stream)
+}  // extern "C"
+
+namespace {
+
+// State of the CUDA-shaped H.v kept between calls: the rank table and NCmat
+// of the last (N, M), the folded coefficients of the last Ho/Hint (reused
+// while the caller's device arrays hold the same values entry for entry)
+// and int64 copies of int32 strides.  One call at a time (g_dev_mu).
+struct DevShapeCache {
+    int N = -1, M = -1;
+    int32_t *B = nullptr, *NCmat = nullptr;
+    std::vector<cplx> ho, hint;
+    HamCoefDev c{};
+    void *block = nullptr;
+    int64_t *w_ot = nullptr, *w_tt = nullptr;
+    int64_t nw = 0;
+};
+DevShapeCache g_devc;
+std::mutex g_dev_mu;
+
+mcpi_status dev_shape_apply(int N, int M, const int32_t *Map, const int32_t *MapOT, const int32_t *MapTT,
+                            const int64_t *strideOT, const int64_t *strideTT, const int32_t *strideOT32,
+                            const int32_t *strideTT32, const int32_t *IF, const cplx *C, const cplx *Ho,
+                            const cplx *Hint, cplx *out, hipStream_t s)
 {
-    MCPI_ARG(Map && MapOT && MapTT && strideOT && strideTT && IF && C && Ho && Hint && out, "null argument");
     MCPI_ARG(M >= 1 && M <= MCPI_MAXM && N >= 0 && N <= MCPI_MAXN, "N or M out of range");
     const int64_t nc = nc_of(N, M);
     MCPI_ARG(nc > 0 && nc <= INT32_MAX, "basis too large");
-    // Ho and Hint are device arrays here: fold them on the host
-    std::vector<cplx> ho((size_t)M * M), hi((size_t)M * M * M * M);
-    MCPI_HIP(hipMemcpy(ho.data(), Ho, sizeof(cplx) * ho.size(), hipMemcpyDeviceToHost));
-    MCPI_HIP(hipMemcpy(hi.data(), Hint, sizeof(cplx) * hi.size(), hipMemcpyDeviceToHost));
-    HamCoefDev c;
-    void *block = nullptr;
-    MCPI_CHECK(build_coefficients(M, ho.data(), hi.data(), &c, &block));
+    std::lock_guard<std::mutex> lk(g_dev_mu);
+    DevShapeCache &g = g_devc;
+    if (g.N != N || g.M != M) {
+        MCPI_HIP(hipDeviceSynchronize());   // earlier launches may still read the old tables
+        if (g.B) (void)hipFree(g.B);
+        if (g.NCmat) (void)hipFree(g.NCmat);
+        g.B = g.NCmat = nullptr;
+        g.N = g.M = -1;
+        MCPI_CHECK(rank_tables_dev(N, M, &g.B, &g.NCmat));
+        g.N = N;
+        g.M = M;
+    }
+    // Ho and Hint are device arrays: read them in stream order and fold them
+    // on the host when they differ from the last call's
+    const size_t nHo = (size_t)M * M, nHi = nHo * nHo;
+    std::vector<cplx> ho(nHo), hi(nHi);
+    MCPI_HIP(hipMemcpyAsync(ho.data(), Ho, sizeof(cplx) * nHo, hipMemcpyDeviceToHost, s));
+    MCPI_HIP(hipMemcpyAsync(hi.data(), Hint, sizeof(cplx) * nHi, hipMemcpyDeviceToHost, s));
+    MCPI_HIP(hipStreamSynchronize(s));
+    if (!g.block || g.ho.size() != nHo || memcmp(g.ho.data(), ho.data(), sizeof(cplx) * nHo) != 0 ||
+        memcmp(g.hint.data(), hi.data(), sizeof(cplx) * nHi) != 0) {
+        MCPI_HIP(hipDeviceSynchronize());
+        if (g.block) (void)hipFree(g.block);
+        g.block = nullptr;
+        g.ho.clear();
+        g.hint.clear();
+        MCPI_CHECK(build_coefficients(M, ho.data(), hi.data(), &g.c, &g.block));
+        g.ho = std::move(ho);
+        g.hint = std::move(hi);
+    }
     BasisDev b;
     memset(&b, 0, sizeof b);
     b.N = N;
@@ -242,31 +301,76 @@ mcpi_status mcpi_hip_applyHconf_dev(int N, int M, const int32_t *Map, const int3
     b.Map = (int32_t *)Map;
     b.MapOT = (int32_t *)MapOT;
     b.MapTT = (int32_t *)MapTT;
+    b.B = g.B;
+    b.NCmat = g.NCmat;
+    // every term evaluated, as the CUDA kernel does.  With a kernel
+    // specialised on M: single jumps read from Map, double-jump targets
+    // ranked from the occupations (the entries MapOT / MapTT hold);
+    // otherwise the generic kernel reads every map and the strides.
+    if (has_fast_kernel(N, M)) return launch_apply_raw(b, MCPI_MAPS_FULL, g.c, 0, 3, C, out, s);
+    if (strideOT32) {
+        if (g.nw < nc) {
+            MCPI_HIP(hipDeviceSynchronize());
+            if (g.w_ot) (void)hipFree(g.w_ot);
+            if (g.w_tt) (void)hipFree(g.w_tt);
+            g.w_ot = g.w_tt = nullptr;
+            g.nw = 0;
+            MCPI_HIP(hipMalloc(&g.w_ot, sizeof(int64_t) * nc));
+            MCPI_HIP(hipMalloc(&g.w_tt, sizeof(int64_t) * nc));
+            g.nw = nc;
+        }
+        MCPI_CHECK(widen_strides(nc, strideOT32, g.w_ot, s));
+        MCPI_CHECK(widen_strides(nc, strideTT32, g.w_tt, s));
+        strideOT = g.w_ot;
+        strideTT = g.w_tt;
+    }
     b.strideOT = (int64_t *)strideOT;
     b.strideTT = (int64_t *)strideTT;
-    mcpi_status st = launch_apply_raw(b, MCPI_MAPS_FULL, c, 0, 1, (const cplx *)C, (cplx *)out,
-                                      (hipStream_t)stream);
-    if (st == MCPI_OK) {
-        hipError_t e = hipStreamSynchronize((hipStream_t)stream);
-        if (e != hipSuccess) st = hip_fail(e, "H.v", __FILE__, __LINE__);
-    }
-    (void)hipFree(block);
-    return st;
+    return launch_apply_raw(b, MCPI_MAPS_FULL, g.c, 0, 1, C, out, s);
+}
+
+}  // namespace
+
+extern "C" {
+
+mcpi_status mcpi_hip_applyHconf_dev(int N, int M, const int32_t *Map, const int32_t *MapOT,
+                                    const int32_t *MapTT, const int64_t *strideOT, const int64_t *strideTT,
+                                    const int32_t *IF, const mcpi_cplx *C, const mcpi_cplx *Ho,
+                                    const mcpi_cplx *Hint, mcpi_cplx *out, void *stream)
+{
+    MCPI_ARG(Map && MapOT && MapTT && strideOT && strideTT && IF && C && Ho && Hint && out, "null argument");
+    return dev_shape_apply(N, M, Map, MapOT, MapTT, strideOT, strideTT, nullptr, nullptr, IF, (const cplx *)C,
+                           (const cplx *)Ho, (const cplx *)Hint, (cplx *)out, (hipStream_t)stream);
+}
+
+mcpi_status mcpi_hip_applyHconf_dev32(int N, int M, const int32_t *Map, const int32_t *MapOT,
+                                      const int32_t *MapTT, const int32_t *strideOT, const int32_t *strideTT,
+                                      const int32_t *IF, const mcpi_cplx *C, const mcpi_cplx *Ho,
+                                      const mcpi_cplx *Hint, mcpi_cplx *out, void *stream)
+{
+    MCPI_ARG(Map && MapOT && MapTT && strideOT && strideTT && IF && C && Ho && Hint && out, "null argument");
+    return dev_shape_apply(N, M, Map, MapOT, MapTT, nullptr, nullptr, strideOT, strideTT, IF, (const cplx *)C,
+                           (const cplx *)Ho, (const cplx *)Hint, (cplx *)out, (hipStream_t)stream);
 }
 
 mcpi_status mcpi_hip_lanczos(mcpi_ham h, int lm, const mcpi_cplx *C0_dev, mcpi_cplx *lvec_dev,
                              mcpi_cplx *diag, mcpi_cplx *offdiag, int *steps)
 {
     MCPI_ARG(h && C0_dev && lvec_dev && diag && offdiag && steps && lm >= 1, "bad argument");
-    return lanczos_api(h, lm, (const cplx *)C0_dev, (cplx *)lvec_dev, (cplx *)diag, (cplx *)offdiag, steps,
-                       lib_stream());
+    std::lock_guard<std::mutex> lk(g_lz_mu);
+    hipStream_t s = nullptr;
+    MCPI_CHECK(lib_stream(&s));
+    return lanczos_api(h, lm, (const cplx *)C0_dev, (cplx *)lvec_dev, (cplx *)diag, (cplx *)offdiag, steps, s);
 }
 
 mcpi_status mcpi_hip_ground(mcpi_ham h, int lm, const mcpi_cplx *C0_dev, mcpi_cplx *C_out_dev, double *E0,
                             int *steps)
 {
     MCPI_ARG(h && C0_dev && C_out_dev && E0 && lm >= 1, "bad argument");
-    return ground_run(h, lm, (const cplx *)C0_dev, (cplx *)C_out_dev, E0, steps, lib_stream(), nullptr);
+    std::lock_guard<std::mutex> lk(g_lz_mu);
+    hipStream_t s = nullptr;
+    MCPI_CHECK(lib_stream(&s));
+    return ground_run(h, lm, (const cplx *)C0_dev, (cplx *)C_out_dev, E0, steps, s, nullptr);
 }
 
 mcpi_status mcpi_hip_tridiag_lowest(int n, const double *d, const double *e, double *lambda, double *z)
@@ -279,8 +383,10 @@ mcpi_status mcpi_hip_set_batched_kernel(int kind) { return set_batched_kernel(ki
 mcpi_status mcpi_hip_lanczos_batched(int n, const mcpi_ham *hams, const mcpi_cplx *const *C0_dev, int lm,
                                      double *E0, int *steps, mcpi_cplx *const *C_out_dev)
 {
-    return lanczos_batched_api(n, hams, (const cplx *const *)C0_dev, lm, E0, steps, (cplx *const *)C_out_dev,
-                               lib_stream());
+    std::lock_guard<std::mutex> lk(g_lz_mu);
+    hipStream_t s = nullptr;
+    MCPI_CHECK(lib_stream(&s));
+    return lanczos_batched_api(n, hams, (const cplx *const *)C0_dev, lm, E0, steps, (cplx *const *)C_out_dev, s);
 }
 
 }  // extern "C"
@@ -362,19 +468,42 @@ std::vector<cplx> flat_ho(int M, Cmatrix Ho)
     return f;
 }
 
-// device basis from caller tables, cached while the same host tables are
-// passed again (a Lanczos loop calling applyHconf_omp every step).  The
-// reference-shaped entry points that use it run one at a time (g_ref_mu,
-// held for the whole call: the cached basis must not be replaced while a
-// call still uses it).
+// Device state of the reference-shaped entry points, kept between calls so
+// that a reference program calling applyHconf_omp inside its own Lanczos
+// loop (groundStateLieb.c:297) or a timing loop (performanceTest.c) pays the
+// table upload, the rule folding and the H.v schedule tuning once:
+//  * the device basis built from the caller's tables, reused while the same
+//    host tables are passed again (same N, M, pointers and a sampled
+//    fingerprint of their contents; tables are taken to be unchanged while
+//    they are passed by the same pointers -- mcpi_hip_release_cached drops
+//    everything after a caller edits tables in place);
+//  * the Hamiltonian of the last call on that basis with its tuned
+//    schedule, reused while Ho and Hint are equal entry for entry to the
+//    copies kept here;
+//  * two device vectors of nc entries.
+// The entry points run one at a time (g_ref_mu, held for the whole call:
+// nothing cached may be replaced while a call still uses it).
 struct RefCache {
     mcpi_basis b = nullptr;
     int N = -1, M = -1;
     const void *key[8] = {};
     uint64_t fp = 0;
+    mcpi_ham h = nullptr;
+    std::vector<cplx> ho, hint;   // flat copies the cached Hamiltonian was built from
+    cplx *dC = nullptr, *dO = nullptr;
+    int64_t nvec = 0;
 };
 RefCache g_cache;
 std::recursive_mutex g_ref_mu;
+
+void drop_cache()
+{
+    if (g_cache.h) mcpi_hip_ham_destroy(g_cache.h);
+    if (g_cache.dC) (void)hipFree(g_cache.dC);
+    if (g_cache.dO) (void)hipFree(g_cache.dO);
+    if (g_cache.b) mcpi_hip_basis_destroy(g_cache.b);
+    g_cache = RefCache();
+}
 
 uint64_t fingerprint(const int *a, int64_t n)
 {
@@ -386,25 +515,31 @@ uint64_t fingerprint(const int *a, int64_t n)
     return h;
 }
 
+// IF == nullptr: a basis generated on the device from (N, M) alone
 mcpi_basis ref_basis(int N, int M, const int *IF, const int *NCmat, const int *Map, const int *MapOT,
                      const int *MapTT, const int *strideOT, const int *strideTT)
 {
     const int nc = ref_nc(N, M);
-    int64_t n_ot = 0, n_tt = 0;
-    count_chunks(M, nc, IF, &n_ot, &n_tt);
     const int64_t MM = (int64_t)M * M;
-    uint64_t fp = fingerprint(IF, (int64_t)nc * M) ^ (fingerprint(Map, Map ? nc * MM : 0) * 3) ^
-                  (fingerprint(MapOT, MapOT ? n_ot * MM : 0) * 5) ^ (fingerprint(MapTT, MapTT ? n_tt * MM : 0) * 7) ^
-                  (fingerprint(strideOT, strideOT ? nc : 0) * 11) ^ (fingerprint(strideTT, strideTT ? nc : 0) * 13);
-    const void *key[8] = {IF, NCmat, Map, MapOT, MapTT, strideOT, strideTT, nullptr};
+    int64_t n_ot = 0, n_tt = 0;
+    uint64_t fp = 0;
+    if (IF) {
+        count_chunks(M, nc, IF, &n_ot, &n_tt);
+        fp = fingerprint(IF, (int64_t)nc * M) ^ (fingerprint(Map, Map ? nc * MM : 0) * 3) ^
+             (fingerprint(MapOT, MapOT ? n_ot * MM : 0) * 5) ^ (fingerprint(MapTT, MapTT ? n_tt * MM : 0) * 7) ^
+             (fingerprint(strideOT, strideOT ? nc : 0) * 11) ^ (fingerprint(strideTT, strideTT ? nc : 0) * 13);
+    }
+    const void *key[8] = {IF, NCmat, Map, MapOT, MapTT, strideOT, strideTT, IF ? nullptr : (const void *)&g_cache};
     // the caller holds g_ref_mu
     if (g_cache.b && g_cache.N == N && g_cache.M == M && g_cache.fp == fp &&
         memcmp(g_cache.key, key, sizeof key) == 0)
         return g_cache.b;
-    if (g_cache.b) mcpi_hip_basis_destroy(g_cache.b);
-    g_cache.b = nullptr;
+    drop_cache();
     mcpi_basis b = nullptr;
-    ref_check(basis_upload_raw(N, M, IF, NCmat, Map, MapOT, MapTT, strideOT, strideTT, n_ot * MM, n_tt * MM, &b));
+    if (IF)
+        ref_check(basis_upload_raw(N, M, IF, NCmat, Map, MapOT, MapTT, strideOT, strideTT, n_ot * MM, n_tt * MM, &b));
+    else
+        ref_check(mcpi_hip_basis_create(N, M, MCPI_MAPS_NONE, &b));
     g_cache.b = b;
     g_cache.N = N;
     g_cache.M = M;
@@ -413,28 +548,79 @@ mcpi_basis ref_basis(int N, int M, const int *IF, const int *NCmat, const int *M
     return b;
 }
 
-struct DevVec {
-    cplx *p = nullptr;
-    explicit DevVec(int64_t n)
-    {
-        if (hipMalloc(&p, sizeof(cplx) * (size_t)std::max<int64_t>(n, 1)) != hipSuccess)
+// the Hamiltonian of (Ho, Hint) on the cached basis b
+mcpi_ham ref_ham(mcpi_basis b, Cmatrix Ho, const Carray Hint)
+{
+    const int M = b->d.M;
+    std::vector<cplx> ho = flat_ho(M, Ho);
+    const size_t nhi = (size_t)M * M * M * M;
+    if (g_cache.h && g_cache.h->basis == b && g_cache.hint.size() == nhi &&
+        memcmp(g_cache.ho.data(), ho.data(), sizeof(cplx) * ho.size()) == 0 &&
+        memcmp(g_cache.hint.data(), Hint, sizeof(cplx) * nhi) == 0)
+        return g_cache.h;
+    if (g_cache.h) mcpi_hip_ham_destroy(g_cache.h);
+    g_cache.h = nullptr;
+    mcpi_ham h = nullptr;
+    ref_check(mcpi_hip_ham_create(b, (const mcpi_cplx *)ho.data(), Hint, &h));
+    g_cache.h = h;
+    g_cache.ho = std::move(ho);
+    g_cache.hint.assign((const cplx *)Hint, (const cplx *)Hint + nhi);
+    return h;
+}
+
+// the two cached device vectors, nc entries each
+void ref_vectors(int64_t nc, cplx **dC, cplx **dO)
+{
+    if (g_cache.nvec != nc) {
+        if (g_cache.dC) (void)hipFree(g_cache.dC);
+        if (g_cache.dO) (void)hipFree(g_cache.dO);
+        g_cache.dC = g_cache.dO = nullptr;
+        g_cache.nvec = 0;
+        const size_t bytes = sizeof(cplx) * (size_t)std::max<int64_t>(nc, 1);
+        if (hipMalloc(&g_cache.dC, bytes) != hipSuccess || hipMalloc(&g_cache.dO, bytes) != hipSuccess)
             ref_die("GPU MEMORY ERROR : hipMalloc fail for complex");
+        g_cache.nvec = nc;
     }
-    ~DevVec() { (void)hipFree(p); }
-};
+    *dC = g_cache.dC;
+    if (dO) *dO = g_cache.dO;
+}
+
+void ref_upload(cplx *d, const Carray C, int64_t nc)
+{
+    if (hipMemcpy(d, C, sizeof(cplx) * nc, hipMemcpyHostToDevice) != hipSuccess) ref_die("GPU copy failed");
+}
 
 void ref_apply(mcpi_basis b, const Carray C, Cmatrix Ho, const Carray Hint, Carray out)
 {
+    const int64_t nc = b->d.nc;
+    mcpi_ham h = ref_ham(b, Ho, Hint);
+    cplx *dC = nullptr, *dO = nullptr;
+    ref_vectors(nc, &dC, &dO);
+    ref_upload(dC, C, nc);
+    ref_check(launch_apply(h, dC, dO, nullptr));
+    if (hipMemcpy(out, dO, sizeof(cplx) * nc, hipMemcpyDeviceToHost) != hipSuccess) ref_die("GPU copy failed");
+}
+
+void ref_rho1(mcpi_basis b, const Carray C, Cmatrix rho)
+{
     const int M = b->d.M;
     const int64_t nc = b->d.nc;
-    std::vector<cplx> ho = flat_ho(M, Ho);
-    mcpi_ham h = nullptr;
-    ref_check(mcpi_hip_ham_create(b, (const mcpi_cplx *)ho.data(), Hint, &h));
-    DevVec dC(nc), dO(nc);
-    if (hipMemcpy(dC.p, C, sizeof(cplx) * nc, hipMemcpyHostToDevice) != hipSuccess) ref_die("GPU copy failed");
-    ref_check(launch_apply(h, dC.p, dO.p, nullptr));
-    if (hipMemcpy(out, dO.p, sizeof(cplx) * nc, hipMemcpyDeviceToHost) != hipSuccess) ref_die("GPU copy failed");
-    mcpi_hip_ham_destroy(h);
+    cplx *dC = nullptr;
+    ref_vectors(nc, &dC, nullptr);
+    ref_upload(dC, C, nc);
+    std::vector<cplx> r((size_t)M * M);
+    ref_check(mcpi_hip_rho1(b, (const mcpi_cplx *)dC, (mcpi_cplx *)r.data()));
+    for (int k = 0; k < M; k++)
+        for (int l = 0; l < M; l++) memcpy(&rho[k][l], &r[k + (size_t)M * l], sizeof(cplx));
+}
+
+void ref_rho2(mcpi_basis b, const Carray C, Carray rho)
+{
+    const int64_t nc = b->d.nc;
+    cplx *dC = nullptr;
+    ref_vectors(nc, &dC, nullptr);
+    ref_upload(dC, C, nc);
+    ref_check(mcpi_hip_rho2(b, (const mcpi_cplx *)dC, rho));
 }
 
 }  // namespace
@@ -549,6 +735,19 @@ void applyHconf_XX(int N, int M, Iarray Map, Iarray MapOT, Iarray MapTT, Iarray 
     applyHconf_omp(N, M, Map, MapOT, MapTT, strideOT, strideTT, IF, C, Ho, Hint, out);
 }
 
+void applyHconf_X(int N, int M, Iarray Map, Iarray MapOT, Iarray strideOT, Iarray NCmat, Iarray IF, Carray C,
+                  Cmatrix Ho, Carray Hint, Carray out)
+{
+    // the two-orbital double jumps are ranked on the fly, as the reference
+    // does here (hamiltonianMatrix.h:730-880); the same-orbital ones too
+    // (same targets as MapOT holds)
+    (void)MapOT;
+    (void)strideOT;
+    std::lock_guard<std::recursive_mutex> lk(g_ref_mu);
+    mcpi_basis b = ref_basis(N, M, IF, NCmat, Map, nullptr, nullptr, nullptr, nullptr);
+    ref_apply(b, C, Ho, Hint, out);
+}
+
 void applyHconf(int N, int M, Iarray NCmat, Iarray IF, Carray C, Cmatrix Ho, Carray Hint, Carray out)
 {
     std::lock_guard<std::recursive_mutex> lk(g_ref_mu);
@@ -556,28 +755,46 @@ void applyHconf(int N, int M, Iarray NCmat, Iarray IF, Carray C, Cmatrix Ho, Car
     ref_apply(b, C, Ho, Hint, out);
 }
 
+void OBrho(int N, int M, Iarray NCmat, Carray C, Cmatrix rho)
+{
+    // no table at all: the basis is generated on the device from (N, M)
+    (void)NCmat;
+    std::lock_guard<std::recursive_mutex> lk(g_ref_mu);
+    ref_rho1(ref_basis(N, M, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr), C, rho);
+}
+
+void OBrho_X(int N, int M, Iarray NCmat, Iarray IF, Carray C, Cmatrix rho)
+{
+    std::lock_guard<std::recursive_mutex> lk(g_ref_mu);
+    ref_rho1(ref_basis(N, M, IF, NCmat, nullptr, nullptr, nullptr, nullptr, nullptr), C, rho);
+}
+
 void OBrho_XM(int N, int M, Iarray Map, Iarray NCmat, Iarray IF, Carray C, Cmatrix rho)
 {
     std::lock_guard<std::recursive_mutex> lk(g_ref_mu);
-    mcpi_basis b = ref_basis(N, M, IF, NCmat, Map, nullptr, nullptr, nullptr, nullptr);
-    const int64_t nc = b->d.nc;
-    DevVec dC(nc);
-    if (hipMemcpy(dC.p, C, sizeof(cplx) * nc, hipMemcpyHostToDevice) != hipSuccess) ref_die("GPU copy failed");
-    std::vector<cplx> r((size_t)M * M);
-    ref_check(mcpi_hip_rho1(b, (const mcpi_cplx *)dC.p, (mcpi_cplx *)r.data()));
-    for (int k = 0; k < M; k++)
-        for (int l = 0; l < M; l++) memcpy(&rho[k][l], &r[k + (size_t)M * l], sizeof(cplx));
+    ref_rho1(ref_basis(N, M, IF, NCmat, Map, nullptr, nullptr, nullptr, nullptr), C, rho);
+}
+
+void TBrho(int N, int M, Iarray NCmat, Iarray IF, Carray C, Carray rho)
+{
+    std::lock_guard<std::recursive_mutex> lk(g_ref_mu);
+    ref_rho2(ref_basis(N, M, IF, NCmat, nullptr, nullptr, nullptr, nullptr, nullptr), C, rho);
+}
+
+void TBrho_X(int N, int M, Iarray Map, Iarray MapOT, Iarray strideOT, Iarray NCmat, Iarray IF, Carray C,
+             Carray rho)
+{
+    (void)MapOT;
+    (void)strideOT;
+    std::lock_guard<std::recursive_mutex> lk(g_ref_mu);
+    ref_rho2(ref_basis(N, M, IF, NCmat, Map, nullptr, nullptr, nullptr, nullptr), C, rho);
 }
 
 void TBrho_XX(int N, int M, Iarray Map, Iarray MapOT, Iarray MapTT, Iarray strideOT, Iarray strideTT,
               Iarray NCmat, Iarray IF, Carray C, Carray rho)
 {
     std::lock_guard<std::recursive_mutex> lk(g_ref_mu);
-    mcpi_basis b = ref_basis(N, M, IF, NCmat, Map, MapOT, MapTT, strideOT, strideTT);
-    const int64_t nc = b->d.nc;
-    DevVec dC(nc);
-    if (hipMemcpy(dC.p, C, sizeof(cplx) * nc, hipMemcpyHostToDevice) != hipSuccess) ref_die("GPU copy failed");
-    ref_check(mcpi_hip_rho2(b, (const mcpi_cplx *)dC.p, rho));
+    ref_rho2(ref_basis(N, M, IF, NCmat, Map, MapOT, MapTT, strideOT, strideTT), C, rho);
 }
 
 int lanczos(int lm, int Npar, int Morb, Iarray IF, Iarray strideOT, Iarray strideTT, Iarray Map, Iarray MapOT,
@@ -586,19 +803,16 @@ int lanczos(int lm, int Npar, int Morb, Iarray IF, Iarray strideOT, Iarray strid
     std::lock_guard<std::recursive_mutex> lk(g_ref_mu);
     mcpi_basis b = ref_basis(Npar, Morb, IF, nullptr, Map, MapOT, MapTT, strideOT, strideTT);
     const int64_t nc = b->d.nc;
-    std::vector<cplx> ho = flat_ho(Morb, Ho);
-    mcpi_ham h = nullptr;
-    ref_check(mcpi_hip_ham_create(b, (const mcpi_cplx *)ho.data(), Hint, &h));
+    mcpi_ham h = ref_ham(b, Ho, Hint);
     cplx *dl = nullptr;
     if (hipMalloc(&dl, sizeof(cplx) * nc * (size_t)lm) != hipSuccess) ref_die("GPU MEMORY ERROR : Lanczos basis");
-    if (hipMemcpy(dl, lvec[0], sizeof(cplx) * nc, hipMemcpyHostToDevice) != hipSuccess) ref_die("GPU copy failed");
+    ref_upload(dl, lvec[0], nc);
     int steps = 0;
     ref_check(mcpi_hip_lanczos(h, lm, (const mcpi_cplx *)dl, (mcpi_cplx *)dl, diag, offdiag, &steps));
     for (int t = 0; t < steps; t++)
         if (hipMemcpy(lvec[t], dl + (size_t)t * nc, sizeof(cplx) * nc, hipMemcpyDeviceToHost) != hipSuccess)
             ref_die("GPU copy failed");
     (void)hipFree(dl);
-    mcpi_hip_ham_destroy(h);
     return steps;
 }
 
@@ -608,17 +822,22 @@ double ground(int Niter, int Npar, int Morb, Iarray IF, Iarray strideOT, Iarray 
     std::lock_guard<std::recursive_mutex> lk(g_ref_mu);
     mcpi_basis b = ref_basis(Npar, Morb, IF, nullptr, Map, MapOT, MapTT, strideOT, strideTT);
     const int64_t nc = b->d.nc;
-    std::vector<cplx> ho = flat_ho(Morb, Ho);
-    mcpi_ham h = nullptr;
-    ref_check(mcpi_hip_ham_create(b, (const mcpi_cplx *)ho.data(), Hint, &h));
-    DevVec dC(nc);
-    if (hipMemcpy(dC.p, C, sizeof(cplx) * nc, hipMemcpyHostToDevice) != hipSuccess) ref_die("GPU copy failed");
+    mcpi_ham h = ref_ham(b, Ho, Hint);
+    cplx *dC = nullptr;
+    ref_vectors(nc, &dC, nullptr);
+    ref_upload(dC, C, nc);
     double E0 = 0;
     int steps = 0;
-    ref_check(mcpi_hip_ground(h, Niter, (const mcpi_cplx *)dC.p, (mcpi_cplx *)dC.p, &E0, &steps));
-    if (hipMemcpy(C, dC.p, sizeof(cplx) * nc, hipMemcpyDeviceToHost) != hipSuccess) ref_die("GPU copy failed");
-    mcpi_hip_ham_destroy(h);
+    ref_check(mcpi_hip_ground(h, Niter, (const mcpi_cplx *)dC, (mcpi_cplx *)dC, &E0, &steps));
+    if (hipMemcpy(C, dC, sizeof(cplx) * nc, hipMemcpyDeviceToHost) != hipSuccess) ref_die("GPU copy failed");
     return E0;
+}
+
+mcpi_status mcpi_hip_release_cached(void)
+{
+    std::lock_guard<std::recursive_mutex> lk(g_ref_mu);
+    drop_cache();
+    return MCPI_OK;
 }
 
 mcpi_cplx carrDot(int n, Carray v1, Carray v2)

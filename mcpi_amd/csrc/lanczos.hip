@@ -474,56 +474,58 @@ __global__ __launch_bounds__(LZ_T) void lz_ritz(const LzInst *in)
 }
 
 // ------------------------------------------- tridiagonal eigenproblem --
-// Eigenvalues of the symmetric tridiagonal (d, e) by the implicit QL method
-// with Wilkinson-type shifts (the tql1 scheme of EISPACK, Golub & Van Loan
-// sec. 8.3), then the lowest eigenvector by inverse iteration.  Stands in for
-// LAPACKE_dstev (groundStateLieb.c:435).
-static int tridiag_eigvals(int n, double *d, double *e)
+// The reference hands (Re alpha, Re beta) to LAPACKE_dstev and keeps the
+// lowest eigenpair (groundStateLieb.c:435-458).  Only that pair is needed
+// here: the eigenvalue by bisection on Sturm counts, the eigenvector by
+// inverse iteration.
+
+// Eigenvalues of the symmetric tridiagonal T = (d, e) below x: by Sylvester's
+// law of inertia, the negative pivots of the LDL^T factorization of T - xI
+// (Golub & Van Loan, sec. 8.4.1).  e2 holds the squared off-diagonal.  A
+// pivot smaller than pivmin in magnitude is taken as -pivmin (the safeguard
+// of LAPACK's dstebz), which keeps the count monotone in x.
+static int sturm_count(int n, const double *d, const double *e2, double x, double pivmin)
 {
-    if (n <= 0) return 0;
-    e[n - 1] = 0.0;
-    const double eps = 2.220446049250313e-16;
-    for (int l = 0; l < n; l++) {
-        int iter = 0;
-        for (;;) {
-            int m = l;
-            for (; m < n - 1; m++) {
-                double dd = fabs(d[m]) + fabs(d[m + 1]);
-                if (fabs(e[m]) <= eps * dd) break;
-            }
-            if (m == l) break;
-            if (++iter > 60) return -1;
-            double g = (d[l + 1] - d[l]) / (2.0 * e[l]);
-            double r = hypot(g, 1.0);
-            g = d[m] - d[l] + e[l] / (g + (g >= 0 ? fabs(r) : -fabs(r)));
-            double s = 1.0, c = 1.0, p = 0.0;
-            int i;
-            bool underflow = false;
-            for (i = m - 1; i >= l; i--) {
-                double f = s * e[i], b = c * e[i];
-                r = hypot(f, g);
-                e[i + 1] = r;
-                if (r == 0.0) {
-                    d[i + 1] -= p;
-                    e[m] = 0.0;
-                    underflow = true;
-                    break;
-                }
-                s = f / r;
-                c = g / r;
-                g = d[i + 1] - p;
-                r = (d[i] - g) * s + 2.0 * c * b;
-                p = s * r;
-                d[i + 1] = g + p;
-                g = c * r - b;
-            }
-            if (underflow) continue;
-            d[l] -= p;
-            e[l] = g;
-            e[m] = 0.0;
-        }
+    int below = 0;
+    double piv = 1.0;
+    for (int i = 0; i < n; i++) {
+        piv = (d[i] - x) - (i ? e2[i - 1] / piv : 0.0);
+        if (fabs(piv) < pivmin) piv = -pivmin;
+        below += piv < 0.0;
     }
-    return 0;
+    return below;
+}
+
+// Lowest eigenvalue of T by bisection: count(lo) = 0 and count(hi) >= 1 are
+// kept while the bracket halves, until lo and hi are neighbouring doubles.
+// Start: a Gershgorin lower bound and the smallest diagonal entry (every
+// d_i is a Rayleigh quotient, so lambda_min <= min d), each widened by a
+// few ulps so that the two counts hold at the start.
+static double lowest_eigenvalue(int n, const double *d, const double *e)
+{
+    std::vector<double> e2(n > 1 ? n - 1 : 1, 0.0);
+    double big = 1.0;
+    for (int i = 0; i + 1 < n; i++) {
+        e2[i] = e[i] * e[i];
+        big = std::max(big, e2[i]);
+    }
+    const double pivmin = 2.2250738585072014e-308 * big;
+    double lo = d[0], hi = d[0];
+    for (int i = 0; i < n; i++) {
+        const double radius = (i > 0 ? fabs(e[i - 1]) : 0.0) + (i + 1 < n ? fabs(e[i]) : 0.0);
+        lo = std::min(lo, d[i] - radius);
+        hi = std::min(hi, d[i]);
+    }
+    const double pad = 4.0 * 2.220446049250313e-16 * std::max(fabs(lo), fabs(hi)) + pivmin;
+    lo -= pad;
+    hi += pad;
+    for (int it = 0; it < 4096; it++) {
+        const double mid = lo + 0.5 * (hi - lo);
+        if (!(mid > lo && mid < hi)) break;
+        if (sturm_count(n, d, e2.data(), mid, pivmin) >= 1) hi = mid;
+        else lo = mid;
+    }
+    return lo + 0.5 * (hi - lo);
 }
 
 static void tridiag_inverse_iteration(int n, const double *d, const double *e, double lam, double *z)
@@ -587,20 +589,16 @@ static void tridiag_inverse_iteration(int n, const double *d, const double *e, d
 mcpi_status tridiag_lowest(int n, const double *d, const double *e, double *lambda, double *z)
 {
     MCPI_ARG(n >= 1 && d && lambda, "bad tridiagonal");
-    std::vector<double> dd(d, d + n), ee(n, 0.0);
+    std::vector<double> ee(n, 0.0);
     for (int i = 0; i < n - 1; i++) ee[i] = e ? e[i] : 0.0;
-    if (tridiag_eigvals(n, dd.data(), ee.data()) != 0) {
-        set_error("tridiagonal QL did not converge");
-        return MCPI_ERR_STATE;
-    }
-    double lam = dd[0];
-    for (int i = 1; i < n; i++) lam = std::min(lam, dd[i]);
+    for (int i = 0; i < n; i++)
+        if (!std::isfinite(d[i]) || !std::isfinite(ee[i])) {
+            set_error("tridiagonal has a non-finite entry");
+            return MCPI_ERR_STATE;
+        }
+    const double lam = lowest_eigenvalue(n, d, ee.data());
     *lambda = lam;
-    if (z) {
-        std::vector<double> ev(n - 1 > 0 ? n - 1 : 1, 0.0);
-        for (int i = 0; i < n - 1; i++) ev[i] = e ? e[i] : 0.0;
-        tridiag_inverse_iteration(n, d, ev.data(), lam, z);
-    }
+    if (z) tridiag_inverse_iteration(n, d, ee.data(), lam, z);
     return MCPI_OK;
 }
 

@@ -268,6 +268,11 @@ mcpi_status basis_upload_raw(int N, int M, const int32_t *IF, const int32_t *NCm
                              const int32_t *Map, const int32_t *MapOT, const int32_t *MapTT,
                              const int *strideOT, const int *strideTT, int64_t ot_len,
                              int64_t tt_len, mcpi_basis *out);
+// rank table B and NCmat of (N, M) in device memory, for kernels that run on
+// caller-owned tables (free both with hipFree)
+mcpi_status rank_tables_dev(int N, int M, int32_t **B, int32_t **NCmat);
+// w[0 .. n) = a[0 .. n) widened to int64, on stream st
+mcpi_status widen_strides(int64_t n, const int32_t *a, int64_t *w, hipStream_t st);
 // build the requested maps of a basis whose IF table is in place
 mcpi_status basis_build_parts(mcpi_basis b, int want_map, int want_ot, int want_tt);
 

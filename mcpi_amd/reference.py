@@ -123,6 +123,42 @@ class RefBasis:
                         Hm.ptr, _vp(Hf.ctypes.data), _vp(out.ctypes.data))
         return out
 
+    def applyHconf_X(self, C, Ho, Hint) -> np.ndarray:
+        C = np.ascontiguousarray(C, np.complex128)
+        Hf = flat(Hint)
+        out = np.zeros(self.nc, np.complex128)
+        Hm = _Cmatrix(Ho)
+        _L().applyHconf_X(self.N, self.M, _ip_of(self.Map), _ip_of(self._nz(self.MapOT)),
+                          _ip_of(self.strideOT), _ip_of(self.NCmat), _ip_of(self.IF), _vp(C.ctypes.data),
+                          Hm.ptr, _vp(Hf.ctypes.data), _vp(out.ctypes.data))
+        return out
+
+    def OBrho(self, C) -> np.ndarray:
+        C = np.ascontiguousarray(C, np.complex128)
+        R = _Cmatrix(np.zeros((self.M, self.M), np.complex128))
+        _L().OBrho(self.N, self.M, _ip_of(self.NCmat), _vp(C.ctypes.data), R.ptr)
+        return R.rows.copy()
+
+    def OBrho_X(self, C) -> np.ndarray:
+        C = np.ascontiguousarray(C, np.complex128)
+        R = _Cmatrix(np.zeros((self.M, self.M), np.complex128))
+        _L().OBrho_X(self.N, self.M, _ip_of(self.NCmat), _ip_of(self.IF), _vp(C.ctypes.data), R.ptr)
+        return R.rows.copy()
+
+    def TBrho(self, C) -> np.ndarray:
+        C = np.ascontiguousarray(C, np.complex128)
+        out = np.zeros(self.M ** 4, np.complex128)
+        _L().TBrho(self.N, self.M, _ip_of(self.NCmat), _ip_of(self.IF), _vp(C.ctypes.data),
+                   _vp(out.ctypes.data))
+        return unflat(out, self.M, 4)
+
+    def TBrho_X(self, C) -> np.ndarray:
+        C = np.ascontiguousarray(C, np.complex128)
+        out = np.zeros(self.M ** 4, np.complex128)
+        _L().TBrho_X(self.N, self.M, _ip_of(self.Map), _ip_of(self._nz(self.MapOT)), _ip_of(self.strideOT),
+                     _ip_of(self.NCmat), _ip_of(self.IF), _vp(C.ctypes.data), _vp(out.ctypes.data))
+        return unflat(out, self.M, 4)
+
     def OBrho_XM(self, C) -> np.ndarray:
         C = np.ascontiguousarray(C, np.complex128)
         R = _Cmatrix(np.zeros((self.M, self.M), np.complex128))
@@ -175,6 +211,12 @@ def setupHint(M: int, g: float) -> np.ndarray:
     out = np.zeros(M ** 4, np.complex128)
     _L().setupHint(M, ct.c_double(g), _vp(out.ctypes.data))
     return unflat(out, M, 4)
+
+
+def release_cached() -> None:
+    """Drop the device state the reference-shaped entry points keep between
+    calls (mcpi_hip_release_cached)."""
+    _L().mcpi_hip_release_cached()
 
 
 def carrMod(v) -> float:

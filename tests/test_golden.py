@@ -24,6 +24,30 @@ def test_oracle_matches_golden(oracle_lib, path):
         assert np.array_equal(b.strideOT, g["strideOT"]) and np.array_equal(b.strideTT, g["strideTT"])
     out = b.acth(g["C"], g["Ho"], g["Hint"])
     assert np.linalg.norm(out - g["out"]) <= 1e-13 * np.linalg.norm(g["out"])
-    assert np.linalg.norm(out - g["out_hash"]) <= 1e-13 * np.linalg.norm(g["out"])
-    assert np.abs(b.rho1(g["C"]) - g["rho1"]).max() <= 1e-13
-    assert np.abs(b.rho2(g["C"]) - g["rho2"]).max() <= 1e-12
+    if "out_hash" in g:
+        assert np.linalg.norm(out - g["out_hash"]) <= 1e-13 * np.linalg.norm(g["out"])
+    if "rho1" in g:
+        assert np.abs(b.rho1(g["C"]) - g["rho1"]).max() <= 1e-13
+        assert np.abs(b.rho2(g["C"]) - g["rho2"]).max() <= 1e-12
+
+
+def test_weak_symmetry_splits_the_reference_variants(oracle_lib):
+    """With a Hint that has only Hint[k,l,q,s] = Hint[l,k,s,q] =
+    conj(Hint[q,s,k,l]), the reference's folded rules (applyHconf_omp,
+    hamiltonianMatrix.h:1431) and its hash-only applyHconf (:31, which sums
+    the two-orbital terms over all ordered index pairs) give different
+    operators; the oracle, like the GPU path, follows applyHconf_omp.  Both
+    are still Hermitian: <x, H y> = conj(<y, H x>) for the folded operator."""
+    g = np.load(os.path.join(GOLD, "cfg1_weaksym.npz"))
+    H = g["Hint"]
+    assert np.abs(H - H.transpose(1, 0, 3, 2)).max() < 1e-15
+    assert np.abs(H - np.conj(H.transpose(2, 3, 0, 1))).max() < 1e-15
+    assert np.abs(H - H.transpose(1, 0, 2, 3)).max() > 0.1      # not fully symmetric
+    d = np.linalg.norm(g["out_hash_variant"] - g["out"]) / np.linalg.norm(g["out"])
+    assert d > 1e-2
+    b = oracle.Basis(5, 5)
+    rng = np.random.default_rng(7)
+    x = rng.standard_normal(b.nc) + 1j * rng.standard_normal(b.nc)
+    y = rng.standard_normal(b.nc) + 1j * rng.standard_normal(b.nc)
+    hx, hy = b.acth(x, g["Ho"], H), b.acth(y, g["Ho"], H)
+    assert abs(np.vdot(x, hy) - np.conj(np.vdot(y, hx))) < 1e-12 * np.linalg.norm(hx) * np.linalg.norm(y)

@@ -116,6 +116,52 @@ def test_reference_shaped_apply(oracle_lib):
         assert_close(r.applyHconf(C, Ho, Hint), ref, 1e-12, "applyHconf")
 
 
+def test_reference_shaped_variants_and_cache(oracle_lib):
+    """applyHconf_X (hamiltonianMatrix.h:431) and the repeated-call cache:
+    the same arrays again reuse the device tables and the tuned Hamiltonian;
+    a changed Hint (same array, new values) or new arrays are picked up; the
+    cache can be dropped."""
+    N, M = 5, 6
+    note("reference-shaped applyHconf_X and the call cache")
+    r = R.RefBasis(N, M)
+    o = oracle.Basis(N, M)
+    C, Ho, Hint = oracle.random_inputs(N, M, seed=15)
+    ref = o.acth(C, Ho, Hint)
+    assert_close(r.applyHconf_X(C, Ho, Hint), ref, 1e-12, "applyHconf_X")
+    for _ in range(3):
+        assert_close(r.applyHconf_omp(C, Ho, Hint), ref, 1e-12, "applyHconf_omp repeated")
+    Hint2 = Hint.copy()
+    Hint2 *= 0.5
+    assert_close(r.applyHconf_omp(C, Ho, Hint2), o.acth(C, Ho, Hint2), 1e-12, "Hint changed")
+    Ho2 = Ho + np.eye(M)
+    assert_close(r.applyHconf_omp(C, Ho2, Hint2), o.acth(C, Ho2, Hint2), 1e-12, "Ho changed")
+    R.release_cached()
+    assert_close(r.applyHconf_omp(C, Ho, Hint), ref, 1e-12, "after release")
+    r2 = R.RefBasis(4, 7)
+    o2 = oracle.Basis(4, 7)
+    C2, Ho3, Hint3 = oracle.random_inputs(4, 7, seed=16)
+    assert_close(r2.applyHconf_omp(C2, Ho3, Hint3), o2.acth(C2, Ho3, Hint3), 1e-12, "other basis")
+
+
+def test_reference_shaped_weak_symmetry_fixture():
+    """cfg1 with a Hint of only the weaker symmetry
+    Hint[k,l,q,s] = Hint[l,k,s,q] = conj(Hint[q,s,k,l]): the reference's
+    applyHconf_omp output (written by its own build) through the
+    reference-shaped entry point, every schedule of the extended API, and
+    the CUDA-shaped one."""
+    gd = np.load(os.path.join(os.path.dirname(__file__), "golden", "cfg1_weaksym.npz"))
+    N, M = int(gd["N"]), int(gd["M"])
+    note("weak-symmetry cfg1 fixture")
+    r = R.RefBasis(N, M)
+    assert_close(r.applyHconf_omp(gd["C"], gd["Ho"], gd["Hint"]), gd["out"], 1e-12, "applyHconf_omp")
+    assert_close(r.applyHconf_X(gd["C"], gd["Ho"], gd["Hint"]), gd["out"], 1e-12, "applyHconf_X")
+    b = mcpi_amd.Basis(N, M, "full")
+    for sched in schedules(M, "full"):
+        for skip in (True, False):
+            H = mcpi_amd.Hamiltonian(b, gd["Ho"], gd["Hint"], skip_zero=skip, schedule=sched)
+            assert_close(H(gd["C"]), gd["out"], 1e-12, f"schedule {sched} skip {skip}")
+
+
 def test_cuda_shaped_dev_entry_point(oracle_lib):
     N, M = 6, 5
     note("applyHconf_dev N=6 M=5")
@@ -127,7 +173,32 @@ def test_cuda_shaped_dev_entry_point(oracle_lib):
     out = mcpi_amd.DeviceArray(o.nc)
     mcpi_amd.applyHconf_dev(N, M, D(o.Map), D(o.MapOT), D(o.MapTT), D(s_ot), D(s_tt), D(o.IF),
                             D(C), D(oracle.flat(Ho)), D(oracle.flat(Hint)), out)
+    mcpi_amd.synchronize()
     assert_close(out.numpy(), o.acth(C, Ho, Hint), 1e-12)
+
+
+@pytest.mark.parametrize("N,M", [(6, 5), (3, 14)])
+def test_cuda_shaped_dev32_entry_point(oracle_lib, N, M):
+    """int32 strides as the CUDA kernel takes them (cuda/hamiltonianMatrix.cuh:
+    61-63); M = 14 runs the generic kernel, which reads the widened strides.
+    Repeated calls reuse the folded coefficients; new Ho/Hint values in the
+    same device arrays are picked up."""
+    note(f"applyHconf_dev32 N={N} M={M}")
+    o = oracle.Basis(N, M)
+    C, Ho, Hint = oracle.random_inputs(N, M, seed=60 + M)
+    D = mcpi_amd.DeviceArray.from_numpy
+    out = mcpi_amd.DeviceArray(o.nc)
+    dHo, dHint = D(oracle.flat(Ho)), D(oracle.flat(Hint))
+    args = (D(o.Map), D(o.MapOT), D(o.MapTT), D(o.strideOT.astype(np.int32)), D(o.strideTT.astype(np.int32)),
+            D(o.IF), D(C))
+    for _ in range(2):
+        mcpi_amd.applyHconf_dev32(N, M, *args, dHo, dHint, out)
+        mcpi_amd.synchronize()
+        assert_close(out.numpy(), o.acth(C, Ho, Hint), 1e-12, "dev32")
+    dHint.upload(oracle.flat(2.0 * Hint))
+    mcpi_amd.applyHconf_dev32(N, M, *args, dHo, dHint, out)
+    mcpi_amd.synchronize()
+    assert_close(out.numpy(), o.acth(C, Ho, 2.0 * Hint), 1e-12, "dev32 new Hint")
 
 
 def _hermitian_check(H, nc, seed):

@@ -90,13 +90,21 @@ def test_lieb_liniger_m11_all(row):
     assert E / N >= bethe.energy_per_particle(N, row["g"]) - 1e-9
 
 
-@pytest.mark.parametrize("g", [1.0, 100.0, 1000.0])
-def test_lieb_liniger_m21(g):
-    """liebGS.dat, 21 orbitals (nc 53,130; 6,641 Lanczos steps)."""
-    row = [r for r in FX["rows"] if r["g"] == g][0]
+@pytest.fixture(scope="module")
+def basis_5_21():
+    return mcpi_amd.Basis(5, 21, "full")
+
+
+@pytest.mark.parametrize("row", FX["rows"], ids=[f"g{r['g']:g}" for r in FX["rows"]])
+def test_lieb_liniger_m21_all(row, basis_5_21):
+    """liebGS.dat, 21 orbitals: every g (nc 53,130; nc/8 = 6,641 Lanczos
+    steps from the reference's start vector)."""
     N, M = 5, 21
-    note(f"liebGS M=21 g={g}")
-    b = mcpi_amd.Basis(N, M, "full")
-    H = mcpi_amd.Hamiltonian(b, mcpi_amd.ll_ho(M), mcpi_amd.ll_hint(M, g))
+    note(f"liebGS M=21 g={row['g']}")
+    b = basis_5_21
+    H = mcpi_amd.Hamiltonian(b, mcpi_amd.ll_ho(M), mcpi_amd.ll_hint(M, row["g"]))
     E, _, steps = H.ground(b.nc // 8, mcpi_amd.start_vector(b.nc))
+    assert steps == b.nc // 8
     assert abs(E / N - row["E_M21"]) <= 5e-6 + 1e-9
+    assert E / N >= bethe.energy_per_particle(N, row["g"]) - 1e-9
+    H.close()

@@ -44,13 +44,22 @@ def test_rho_golden():
 @pytest.mark.parametrize("N,M", [(5, 5), (2, 17)])
 def test_reference_shaped_rho(oracle_lib, N, M):
     """Reference-layout maps passed in; M = 17 reads the double-jump targets
-    from them."""
-    note(f"OBrho_XM / TBrho_XX N={N} M={M}")
+    from them.  Every reference variant: OBrho (no table, onebodyMatrix.h:34),
+    OBrho_X (IF, :116), OBrho_XM (IF, Map, :190), TBrho (IF,
+    twobodyMatrix.h:38), TBrho_X (IF, Map, MapOT, :502), TBrho_XX (:991)."""
+    note(f"OBrho* / TBrho* N={N} M={M}")
     r = R.RefBasis(N, M)
     o = oracle.Basis(N, M)
     C, _, _ = oracle.random_inputs(N, M, seed=1)
-    assert_close(r.OBrho_XM(C), o.rho1(C), 1e-12, "OBrho_XM")
-    assert_close(r.TBrho_XX(C), o.rho2(C), 1e-12, "TBrho_XX")
+    r1, r2 = o.rho1(C), o.rho2(C)
+    assert_close(r.OBrho_XM(C), r1, 1e-12, "OBrho_XM")
+    assert_close(r.TBrho_XX(C), r2, 1e-12, "TBrho_XX")
+    IF0 = r.IF.copy()
+    assert_close(r.OBrho(C), r1, 1e-12, "OBrho")
+    assert_close(r.OBrho_X(C), r1, 1e-12, "OBrho_X")
+    assert_close(r.TBrho(C), r2, 1e-12, "TBrho")
+    assert_close(r.TBrho_X(C), r2, 1e-12, "TBrho_X")
+    assert np.array_equal(r.IF, IF0)
 
 
 def test_rho_invariants_n10_m12(oracle_lib):

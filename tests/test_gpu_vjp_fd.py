@@ -13,7 +13,7 @@ import pytest
 
 import mcpi_amd
 import oracle
-from tests.gpu_util import note
+from tests.gpu_util import assert_close, note, oracle_on_device_tables
 
 pytestmark = pytest.mark.gpu
 
@@ -40,6 +40,13 @@ def test_vjp_central_differences_n10_m12():
     gHo, gHint = b.vjp(psi)
     # energy identity at the eigenvector: E0 = sum Ho.rho1 + 1/2 sum Hint.rho2
     assert abs(np.real(np.sum(gHo * Ho) + np.sum(gHint * Hint)) - E) < 1e-9 * max(1.0, abs(E))
+    # the densities themselves against the oracle's OBrho_XM / TBrho_XX
+    # restatement on the same (device-built) tables at the same vector
+    note("oracle rho1 / rho2 at the ground vector")
+    o = oracle_on_device_tables(b)
+    assert_close(gHo, o.rho1(v), 1e-12, "dE0/dHo vs oracle rho1")
+    assert_close(gHint, 0.5 * o.rho2(v), 1e-12, "dE0/dHint vs oracle rho2 / 2")
+    del o
 
     dirs = [oracle.random_inputs(N, M, seed=7000 + t, sigma=1.0)[1:] for t in range(32)]
     fd = np.zeros(32)

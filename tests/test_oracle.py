@@ -373,3 +373,68 @@ def test_packed_layout_restatement(oracle_lib, N, M):
                         assert got == r
             off += E * len(act)
         assert off == poff[1]
+
+
+def test_lanczos_tridiagonal_is_projection(oracle_lib):
+    """V^H H V = T for the oracle's Lanczos basis (full reorthogonalization):
+    the tridiagonal it returns (Re alpha on the diagonal, Re beta beside it)
+    is the projection of H onto the Krylov basis, to rounding."""
+    N, M, lm = 5, 6, 40
+    b = oracle.Basis(N, M)
+    C0, Ho, Hint = oracle.random_inputs(N, M, seed=31)
+    n, diag, off, V = b.lanczos(lm, C0, Ho, Hint)
+    assert n == lm
+    HV = np.stack([b.acth(V[t], Ho, Hint) for t in range(n)])
+    Tp = V[:n].conj() @ HV.T
+    T = np.diag(diag.real) + np.diag(off.real[: n - 1], 1) + np.diag(off.real[: n - 1], -1)
+    scale = np.abs(T).max()
+    assert np.abs(Tp - T).max() < 1e-11 * scale, np.abs(Tp - T).max() / scale
+    assert np.abs(diag.imag).max() < 1e-12 * scale and np.abs(off.imag).max() == 0.0
+
+
+def test_start_vector_properties(oracle_lib):
+    """groundStateLieb.c:578-586: C_i ~ sin(20 i/nc)(i%13) + I ((i%8)-(i%3)),
+    normalized.  Pinned by properties a transcription slip breaks: unit
+    norm; Re C vanishes at multiples of 13; the imaginary part has period 24,
+    vanishes at multiples of 24 and takes the integer values -2..7 times one
+    unit; Re C_i / (i%13) is A sin(20 i/nc), so
+    f(i+d) + f(i-d) = 2 f(i) cos(20 d/nc)."""
+    nc = 3003
+    C = oracle.start_vector(nc)
+    assert abs(np.linalg.norm(C) - 1) < 1e-14
+    assert np.all(C.real[::13] == 0.0)
+    assert np.array_equal(C.imag[: nc - 24], C.imag[24:])
+    assert np.all(C.imag[::24] == 0.0)
+    u = np.abs(C.imag[C.imag != 0]).min()      # the imaginary parts are integers times u
+    r = C.imag / u
+    assert np.abs(r - np.round(r)).max() < 1e-9 and round(r.min()) == -2 and round(r.max()) == 7
+    d = 7
+    i = np.array([i for i in range(d + 1, nc - d) if i % 13 and (i + d) % 13 and (i - d) % 13])
+    f = lambda j: C.real[j] / (j % 13)  # noqa: E731
+    lhs = f(i + d) + f(i - d)
+    rhs = 2 * f(i) * np.cos(20 * d / nc)
+    assert np.abs(lhs - rhs).max() < 1e-14
+
+
+def test_perf_inputs_properties(oracle_lib):
+    """performanceTest.c:169-219's deterministic Ho and Hint, pinned by what
+    they must satisfy: Ho Hermitian with diagonal (i%4)-1 (period 4, values
+    -1..2); Hint fully bosonic-symmetric (creator swap, annihilator swap,
+    conjugate pair exchange), equal to the 1.234 background wherever the
+    index pairs are not four distinct orbitals, and off it on some disjoint
+    sets."""
+    M = 7
+    Ho = oracle.perf_ho(M)
+    assert np.array_equal(Ho, Ho.conj().T)
+    dg = np.diag(Ho)
+    assert np.all(dg.imag == 0) and np.array_equal(dg.real[:4], [-1, 0, 1, 2])
+    assert np.array_equal(dg[: M - 4], dg[4:])
+    H = oracle.perf_hint(M)
+    assert np.array_equal(H, H.transpose(1, 0, 2, 3))
+    assert np.array_equal(H, H.transpose(0, 1, 3, 2))
+    assert np.array_equal(H, np.conj(H.transpose(2, 3, 0, 1)))
+    idx = np.indices((M,) * 4).reshape(4, -1).T
+    distinct = np.array([len(set(r)) == 4 for r in idx])
+    flatH = H.reshape(-1)
+    assert np.all(flatH[~distinct] == 1.234)
+    assert np.mean(flatH[distinct] != 1.234) > 0.9
