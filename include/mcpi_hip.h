@@ -289,6 +289,16 @@ mcpi_status mcpi_hip_rho2(mcpi_basis b, const mcpi_cplx *C_dev, mcpi_cplx *rho2)
  * dE/dHint = rho2 / 2 (host outputs, flat layouts above) */
 mcpi_status mcpi_hip_vjp(mcpi_basis b, const mcpi_cplx *C_dev, mcpi_cplx *dE_dHo,
                          mcpi_cplx *dE_dHint);
+/* No counterpart in the reference.  For callers that cannot allocate device
+ * memory (code under HIP stream capture, tight serving loops): the adjoint
+ * of mcpi_hip_vjp with device outputs (dE_dHo_dev M*M, dE_dHint_dev M^4),
+ * enqueued on stream (NULL = default) without host synchronization.  Its
+ * partial sums live in a 16 MiB device array of the library (one per GPU,
+ * held until the process ends); it calls no hipMalloc/hipFree.  Bases beyond
+ * the density kernels' limit (M > 24) give MCPI_ERR_ARG.  Calls that
+ * overlap on one GPU share that array and race: serialize them. */
+mcpi_status mcpi_hip_vjp_noalloc(mcpi_basis b, const mcpi_cplx *C_dev, mcpi_cplx *dE_dHo_dev,
+                                 mcpi_cplx *dE_dHint_dev, void *stream);
 
 /* Lanczos on the device.  C0_dev: normalized start vector (nc).  lvec_dev:
  * lm*nc device buffer for the basis (row t = v_t).  diag/offdiag: host, lm

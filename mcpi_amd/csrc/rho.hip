@@ -27,12 +27,14 @@
 //   class 2 (two-orbital double jumps):   R8[ks - p0][ql]   (pairs p0..p1-1)
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <cmath>
 #include <cstring>
 #include <mutex>
 #include <vector>
 
 #include "mcpi_internal.h"
+#include "rho_expand.h"
 
 namespace mcpi {
 
@@ -576,12 +578,89 @@ __global__ void k_sum_blocks(int nblocks, int nterms, const cplx *__restrict__ p
     out[x] = cmake(re, im);
 }
 
-// run one class and bring its compact terms to the host
-template <bool ALL, int R>
-static void launch_rho(const BasisDev &b, const cplx *C, int nterms, cplx *partial, int nblocks, int T, size_t sm)
+// launch geometry of one class
+struct ClassPlan {
+    int T = 1024;       // threads per block
+    int R = 1;          // rows per lane (classes 0 and 3)
+    int nblocks = 1;    // blocks (grid-stride over the rows)
+    size_t sm = 0;      // dynamic LDS bytes
+};
+
+static mcpi_status plan_class(const BasisDev &b, int cls, int nterms, ClassPlan *pl)
 {
-    (void)hipFuncSetAttribute((const void *)k_rho<ALL, R>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)sm);
-    hipLaunchKernelGGL((k_rho<ALL, R>), dim3(nblocks), dim3(T), sm, 0, b, C, nterms, partial);
+    int dev = 0, ncu = 0;
+    MCPI_HIP(hipGetDevice(&dev));
+    MCPI_HIP(hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev));
+    const size_t max_sm = 160 * 1024;
+    const size_t base_sm = sizeof(cplx) * nterms + sizeof(double) * (b.N + 3) + sizeof(int32_t) * b.M * (b.N + 1) + 16;
+    int T = 1024;
+    if (cls == 1 || cls == 2) {
+        // lane per term: up to 16 wavefronts a block, one LDS row tile
+        // (occupations and prefix counts) each; fewer when the accumulators
+        // leave too little LDS (large M)
+        const size_t ms = (size_t)((b.M + 3) & ~3);
+        while (T > 64 && base_sm + 2 * (size_t)T * ms > max_sm) T /= 2;
+        const size_t sm = base_sm + 2 * (size_t)T * ms;
+        MCPI_ARG(sm <= max_sm, "density-matrix accumulators exceed LDS");
+        const int64_t nwv = (b.nc + 63) / 64;
+        pl->T = T;
+        pl->R = 1;
+        pl->sm = sm;
+        pl->nblocks = (int)std::min<int64_t>(std::max<int64_t>((nwv + T / 64 - 1) / (T / 64), 1), (int64_t)ncu);
+        return MCPI_OK;
+    }
+    // classes 0 and 3, rows per lane: as many as the occupation tile leaves
+    // room for, at most 2 (class 0; 4 spills) or 4 (class 3)
+    int R = cls == 0 ? 2 : 4;
+    while (R > 1 && base_sm + (size_t)b.M * T * R > max_sm) R /= 2;
+    // and few enough that every CU gets a tile
+    while (R > 1 && (b.nc + (int64_t)T * R - 1) / ((int64_t)T * R) < ncu) R /= 2;
+    const size_t sm = base_sm + (size_t)b.M * T * R;
+    MCPI_ARG(sm <= max_sm, "density-matrix accumulators exceed LDS");
+    const int64_t tiles = (b.nc + (int64_t)T * R - 1) / ((int64_t)T * R);
+    pl->T = T;
+    pl->R = R;
+    pl->sm = sm;
+    pl->nblocks = (int)std::min<int64_t>(std::max<int64_t>(tiles, 1), (int64_t)ncu);
+    return MCPI_OK;
+}
+
+template <bool ALL, int R>
+static void launch_rho(const BasisDev &b, const cplx *C, int nterms, cplx *partial, const ClassPlan &pl,
+                       hipStream_t s)
+{
+    (void)hipFuncSetAttribute((const void *)k_rho<ALL, R>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)pl.sm);
+    hipLaunchKernelGGL((k_rho<ALL, R>), dim3(pl.nblocks), dim3(pl.T), pl.sm, s, b, C, nterms, partial);
+}
+
+template <int CLASS>
+static void launch_jumps(const BasisDev &b, const cplx *C, int p0, int p1, int nterms, cplx *partial,
+                         const ClassPlan &pl, hipStream_t s)
+{
+    (void)hipFuncSetAttribute((const void *)k_rho_jumps<CLASS>, hipFuncAttributeMaxDynamicSharedMemorySize,
+                              (int)pl.sm);
+    hipLaunchKernelGGL((k_rho_jumps<CLASS>), dim3(pl.nblocks), dim3(pl.T), pl.sm, s, b, C, p0, p1, nterms, partial);
+}
+
+// one class on stream s: block partials into partial (pl.nblocks * nterms),
+// their sums in a fixed order into out (nterms)
+static mcpi_status launch_class(const BasisDev &b, int cls, const cplx *C, int p0, int p1, int nterms,
+                                const ClassPlan &pl, cplx *partial, cplx *out, hipStream_t s)
+{
+    if (cls == 1) launch_jumps<1>(b, C, p0, p1, nterms, partial, pl, s);
+    else if (cls == 2) launch_jumps<2>(b, C, p0, p1, nterms, partial, pl, s);
+    else if (cls == 0) {
+        if (pl.R == 2) launch_rho<true, 2>(b, C, nterms, partial, pl, s);
+        else launch_rho<true, 1>(b, C, nterms, partial, pl, s);
+    } else {
+        if (pl.R == 4) launch_rho<false, 4>(b, C, nterms, partial, pl, s);
+        else if (pl.R == 2) launch_rho<false, 2>(b, C, nterms, partial, pl, s);
+        else launch_rho<false, 1>(b, C, nterms, partial, pl, s);
+    }
+    MCPI_HIP(hipGetLastError());
+    hipLaunchKernelGGL(k_sum_blocks, dim3((nterms + 255) / 256), dim3(256), 0, s, pl.nblocks, nterms, partial, out);
+    MCPI_HIP(hipGetLastError());
+    return MCPI_OK;
 }
 
 // the basis's density scratch: nblocks * nterms block partials, then
@@ -601,68 +680,16 @@ static mcpi_status density_scratch(mcpi_basis bs, int nblocks, int nterms, cplx 
     return MCPI_OK;
 }
 
-template <int CLASS>
-static void launch_jumps(const BasisDev &b, const cplx *C, int p0, int p1, int nterms, cplx *partial, int nblocks,
-                         int T, size_t sm)
-{
-    (void)hipFuncSetAttribute((const void *)k_rho_jumps<CLASS>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)sm);
-    hipLaunchKernelGGL((k_rho_jumps<CLASS>), dim3(nblocks), dim3(T), sm, 0, b, C, p0, p1, nterms, partial);
-}
-
+// run one class and bring its compact terms to the host
 static mcpi_status run_class(mcpi_basis bs, int cls, const cplx *C, int p0, int p1, int nterms,
                              std::vector<cplx> &host)
 {
     const BasisDev &b = bs->d;
-    int T = 1024;
-    int dev = 0, ncu = 0;
-    MCPI_HIP(hipGetDevice(&dev));
-    MCPI_HIP(hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev));
-    const size_t max_sm = 160 * 1024;
-    const size_t base_sm = sizeof(cplx) * nterms + sizeof(double) * (b.N + 3) + sizeof(int32_t) * b.M * (b.N + 1) + 16;
-    if (cls == 1 || cls == 2) {
-        // lane per term: up to 16 wavefronts a block, one LDS row tile
-        // (occupations and prefix counts) each; fewer when the accumulators
-        // leave too little LDS (large M)
-        const size_t ms = (size_t)((b.M + 3) & ~3);
-        while (T > 64 && base_sm + 2 * (size_t)T * ms > max_sm) T /= 2;
-        const size_t sm = base_sm + 2 * (size_t)T * ms;
-        MCPI_ARG(sm <= max_sm, "density-matrix accumulators exceed LDS");
-        const int64_t nwv = (b.nc + 63) / 64;
-        const int nblocks = (int)std::min<int64_t>(std::max<int64_t>((nwv + T / 64 - 1) / (T / 64), 1), (int64_t)ncu);
-        cplx *partial = nullptr, *out = nullptr;
-        MCPI_CHECK(density_scratch(bs, nblocks, nterms, &partial, &out));
-        if (cls == 1) launch_jumps<1>(b, C, p0, p1, nterms, partial, nblocks, T, sm);
-        else launch_jumps<2>(b, C, p0, p1, nterms, partial, nblocks, T, sm);
-        MCPI_HIP(hipGetLastError());
-        hipLaunchKernelGGL(k_sum_blocks, dim3((nterms + 255) / 256), dim3(256), 0, 0, nblocks, nterms, partial, out);
-        MCPI_HIP(hipGetLastError());
-        host.resize(nterms);
-        MCPI_HIP(hipMemcpy(host.data(), out, sizeof(cplx) * nterms, hipMemcpyDeviceToHost));
-        return MCPI_OK;
-    }
-    // classes 0 and 3, rows per lane: as many as the occupation tile leaves
-    // room for, at most 2 (class 0; 4 spills) or 4 (class 3)
-    int R = cls == 0 ? 2 : 4;
-    while (R > 1 && base_sm + (size_t)b.M * T * R > max_sm) R /= 2;
-    // and few enough that every CU gets a tile
-    while (R > 1 && (b.nc + (int64_t)T * R - 1) / ((int64_t)T * R) < ncu) R /= 2;
-    const size_t sm = base_sm + (size_t)b.M * T * R;
-    MCPI_ARG(sm <= max_sm, "density-matrix accumulators exceed LDS");
-    int64_t tiles = (b.nc + (int64_t)T * R - 1) / ((int64_t)T * R);
-    int nblocks = (int)std::min<int64_t>(std::max<int64_t>(tiles, 1), (int64_t)ncu);
+    ClassPlan pl;
+    MCPI_CHECK(plan_class(b, cls, nterms, &pl));
     cplx *partial = nullptr, *out = nullptr;
-    MCPI_CHECK(density_scratch(bs, nblocks, nterms, &partial, &out));
-    if (cls == 0) {
-        if (R == 2) launch_rho<true, 2>(b, C, nterms, partial, nblocks, T, sm);
-        else launch_rho<true, 1>(b, C, nterms, partial, nblocks, T, sm);
-    } else {
-        if (R == 4) launch_rho<false, 4>(b, C, nterms, partial, nblocks, T, sm);
-        else if (R == 2) launch_rho<false, 2>(b, C, nterms, partial, nblocks, T, sm);
-        else launch_rho<false, 1>(b, C, nterms, partial, nblocks, T, sm);
-    }
-    MCPI_HIP(hipGetLastError());
-    hipLaunchKernelGGL(k_sum_blocks, dim3((nterms + 255) / 256), dim3(256), 0, 0, nblocks, nterms, partial, out);
-    MCPI_HIP(hipGetLastError());
+    MCPI_CHECK(density_scratch(bs, pl.nblocks, nterms, &partial, &out));
+    MCPI_CHECK(launch_class(b, cls, C, p0, p1, nterms, pl, partial, out, 0));
     host.resize(nterms);
     MCPI_HIP(hipMemcpy(host.data(), out, sizeof(cplx) * nterms, hipMemcpyDeviceToHost));
     return MCPI_OK;
@@ -768,6 +795,87 @@ static mcpi_status densities(mcpi_basis bs, const cplx *C, cplx *rho1, cplx *rho
     return MCPI_OK;
 }
 
+// ------------------------------------------------ allocation-free adjoint --
+// For callers that cannot allocate device memory (code under stream capture,
+// serving loops): the block partials and the representative sums live in a
+// file-scope device array instead of the basis's scratch, and the full
+// matrices are expanded on the device (rho2_entry), so the call allocates
+// nothing and copies nothing to the host.  Every call on a GPU shares that
+// GPU's array: calls that overlap on one GPU race.
+#define MCPI_NOALLOC_SCRATCH_BYTES (16u << 20)
+static __device__ __align__(256) cplx scratch[MCPI_NOALLOC_SCRATCH_BYTES / sizeof(cplx)];
+
+#define MCPI_NOALLOC_MAXDEV 64
+static std::mutex g_scratch_mu;
+static cplx *g_scratch_dev[MCPI_NOALLOC_MAXDEV];
+
+// device address of this device's copy of `scratch` (looked up once per
+// device, with that device current)
+static mcpi_status noalloc_scratch(cplx **p)
+{
+    int dev = 0;
+    MCPI_HIP(hipGetDevice(&dev));
+    MCPI_ARG(dev >= 0 && dev < MCPI_NOALLOC_MAXDEV, "device ordinal beyond the supported range");
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    if (!g_scratch_dev[dev]) {
+        void *a = nullptr;
+        MCPI_HIP(hipGetSymbolAddress(&a, HIP_SYMBOL(scratch)));
+        g_scratch_dev[dev] = (cplx *)a;
+    }
+    *p = g_scratch_dev[dev];
+    return MCPI_OK;
+}
+
+// rho1 (M*M) and rho2 (M^4) scaled by s2, one thread per entry
+__global__ void k_expand_rho(int M, const cplx *__restrict__ A, const cplx *__restrict__ OB,
+                             const cplx *__restrict__ TT, cplx *__restrict__ rho1, cplx *__restrict__ rho2,
+                             double s2)
+{
+    const int64_t x = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    RepLayout L{M, M * (M - 1) / 2};
+    const int64_t M2 = (int64_t)M * M;
+    if (x < M2) rho1[x] = rho1_entry((int)(x % M), (int)(x / M), L, A);
+    if (x >= M2 * M2) return;
+    const int a = (int)(x % M), b = (int)(x / M % M), c = (int)(x / M2 % M), d = (int)(x / (M2 * M));
+    rho2[x] = cscale(rho2_entry(a, b, c, d, L, A, OB, TT), s2);
+}
+
+static mcpi_status vjp_noalloc(mcpi_basis bs, const cplx *C, cplx *g1, cplx *g2, hipStream_t s)
+{
+    const BasisDev &b = bs->d;
+    const int M = b.M, P = M * (M - 1) / 2;
+    const RepLayout L{M, P};
+    const size_t cap = MCPI_NOALLOC_SCRATCH_BYTES / sizeof(cplx);
+    // representative sums: class 0 | class 1 | class 2 (P x P), then partials
+    const size_t nA = (size_t)L.n_class0(), nOB = (size_t)L.n_class1(), nTT = (size_t)P * P;
+    const size_t reps = (nA + nOB + nTT + 15) & ~(size_t)15;
+    const int chunk = std::max(1, (int)(120 * 1024 / (16 * std::max(P, 1))));
+    const size_t max_terms = std::max({nA, nOB, (size_t)std::min(chunk, std::max(P, 1)) * P});
+    MCPI_ARG(reps + max_terms <= cap,
+             "basis has too many orbitals for the allocation-free adjoint (scratch holds 16 MiB)");
+    cplx *scr = nullptr;
+    MCPI_CHECK(noalloc_scratch(&scr));
+    cplx *A = scr, *OB = A + nA, *TT = OB + nOB, *partial = scr + reps;
+    const size_t room = cap - reps;
+    auto run = [&](int cls, int p0, int p1, int nterms, cplx *out) -> mcpi_status {
+        ClassPlan pl;
+        MCPI_CHECK(plan_class(b, cls, nterms, &pl));
+        pl.nblocks = (int)std::max<size_t>(1, std::min<size_t>((size_t)pl.nblocks, room / (size_t)nterms));
+        return launch_class(b, cls, C, p0, p1, nterms, pl, partial, out, s);
+    };
+    MCPI_CHECK(run(0, 0, 0, (int)nA, A));
+    MCPI_CHECK(run(1, 0, 0, (int)nOB, OB));
+    for (int p0 = 0; p0 < P; p0 += chunk) {
+        const int p1 = std::min(P, p0 + chunk);
+        MCPI_CHECK(run(2, p0, p1, (p1 - p0) * P, TT + (size_t)p0 * P));
+    }
+    const int64_t M4 = (int64_t)M * M * M * M;
+    hipLaunchKernelGGL(k_expand_rho, dim3((unsigned)((M4 + 255) / 256)), dim3(256), 0, s, M, (const cplx *)A,
+                       (const cplx *)OB, (const cplx *)TT, g1, g2, 0.5);
+    MCPI_HIP(hipGetLastError());
+    return MCPI_OK;
+}
+
 }  // namespace mcpi
 
 using namespace mcpi;
@@ -795,6 +903,13 @@ mcpi_status mcpi_hip_vjp(mcpi_basis b, const mcpi_cplx *C_dev, mcpi_cplx *dE_dHo
     cplx *g = (cplx *)dE_dHint;
     for (int64_t x = 0; x < M4; x++) g[x] = cscale(g[x], 0.5);
     return MCPI_OK;
+}
+
+mcpi_status mcpi_hip_vjp_noalloc(mcpi_basis b, const mcpi_cplx *C_dev, mcpi_cplx *dE_dHo_dev,
+                                 mcpi_cplx *dE_dHint_dev, void *stream)
+{
+    MCPI_ARG(b && C_dev && dE_dHo_dev && dE_dHint_dev, "null argument");
+    return vjp_noalloc(b, (const cplx *)C_dev, (cplx *)dE_dHo_dev, (cplx *)dE_dHint_dev, (hipStream_t)stream);
 }
 
 }  // extern "C"
