@@ -42,16 +42,59 @@ def algorithmic_bytes(nc: int, M: int, ot_len: int, tt_len: int) -> int:
     return 16 * nc + 16 * nc + 4 * M * nc + 4 * M * M * nc + 4 * ot_len + 4 * tt_len + 4 * nc + 8 * nc
 
 
+# sources of the headline H.v kernel: the recorded PMC traffic is only used
+# while they are unchanged since it was measured
+KERNEL_SOURCES = ["mcpi_amd/csrc/acth_comp.hip", "mcpi_amd/csrc/acth_fast.h", "mcpi_amd/csrc/mcpi_internal.h"]
+
+
+def kernel_sources_sha() -> str:
+    import hashlib
+    h = hashlib.sha256()
+    for f in KERNEL_SOURCES:
+        h.update(open(os.path.join(ROOT, f), "rb").read())
+    return h.hexdigest()[:16]
+
+
 def traffic_from_profiles():
-    """HBM bytes per H.v launch measured with rocprofv3 PMC counters, if a
-    measurement has been recorded under profiles/."""
+    """HBM bytes per H.v launch measured with rocprofv3 PMC counters
+    (profiles/acth_full_n12_m12_traffic.json), with the record's kernel name
+    and whether the kernel sources still hash to what was measured.  A stale
+    record gives traffic None (the old figure is kept beside it)."""
     p = os.path.join(ROOT, "profiles", "acth_full_n12_m12_traffic.json")
-    if os.path.exists(p):
-        try:
-            return json.load(open(p)).get("hbm_bytes_per_launch")
-        except Exception:
-            return None
-    return None
+    if not os.path.exists(p):
+        return None, {"traffic_record": None}
+    try:
+        rec = json.load(open(p))
+    except Exception:
+        return None, {"traffic_record": "unreadable"}
+    fresh = rec.get("kernel_sources_sha16") == kernel_sources_sha()
+    info = {"traffic_kernel": rec.get("kernel"), "traffic_sources_sha16": rec.get("kernel_sources_sha16"),
+            "traffic_matches_sources": fresh}
+    if not fresh:
+        info["traffic_stale_bytes"] = rec.get("hbm_bytes_per_launch")
+    return (rec.get("hbm_bytes_per_launch") if fresh else None), info
+
+
+def host_cores():
+    """Logical CPUs this process may use, all logical CPUs, and physical cores
+    (distinct (core, socket) pairs in /proc/cpuinfo)."""
+    usable = len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else os.cpu_count()
+    phys = set()
+    try:
+        core = sock = None
+        for ln in open("/proc/cpuinfo"):
+            if ln.startswith("physical id"):
+                sock = ln.split(":")[1].strip()
+            elif ln.startswith("core id"):
+                core = ln.split(":")[1].strip()
+            elif not ln.strip() and core is not None:
+                phys.add((sock, core))
+                core = sock = None
+        if core is not None:
+            phys.add((sock, core))
+    except OSError:
+        pass
+    return {"usable_logical": usable, "logical": os.cpu_count(), "physical": len(phys) or None}
 
 
 def cpu_baseline(b, C, Ho, Hint, gpu_out, target_s: float = 12.0, gpu_rho=None):
@@ -101,14 +144,77 @@ def cpu_baseline(b, C, Ho, Hint, gpu_out, target_s: float = 12.0, gpu_rho=None):
         "value": round(B / t_hv / 1e9, 3),
         "unit": "GB/s",
         "cores": threads,
+        "host_cores": host_cores(),
         "kind": "port",
         "sample": (f"oracle C/OpenMP restatement of applyHconf_omp (full maps, int64 offsets), {len(times)} "
                    f"full H.v over all {nc} rows, {sum(times):.1f} s in total, median {t_hv * 1e3:.1f} ms "
-                   f"per H.v; results match the GPU to {rel:.1e} (relative 2-norm)"),
+                   f"per H.v; results match the GPU to {rel:.1e} (relative 2-norm). The port computes "
+                   f"each two-orbital chunk ordinal once per (row, pair) where the reference recounts it "
+                   f"per term (hamiltonianMatrix.h:1927-1941), so it is faster than the reference's "
+                   f"code; the reference build itself is timed at N=11 under reference_build"),
         "ms_per_hv": round(t_hv * 1e3, 2),
         "table_copy_s": round(copy_s, 2),
         "densities": dens,
     }
+
+
+def reference_build_baseline(target_s: float = 20.0):
+    """The reference's own applyHconf_omp (hamiltonianMatrix.h:1431, compiled
+    unmodified into oracle/_ref) timed on the host cores at N=11, M=12
+    (nc 705,432, Lieb-Liniger g=10): the largest M=12 basis its INT32 guards
+    admit (configurationsMap.h:462-467).  Tables are the device-built ones
+    copied to the host (int32 strides); next to it the oracle's restatement
+    and the GPU on the same basis, all three results compared."""
+    from oracle import ref
+    if not ref.available():
+        return None
+    import mcpi_amd
+    import oracle
+    N, M = 11, 12
+    b = mcpi_amd.Basis(N, M, "full")
+    nc = b.nc
+    IF = b.copy_range("IF", 0, nc * M)
+    Map = b.copy_range("Map", 0, b.map_len)
+    sot = b.copy_range("strideOT", 0, nc)
+    stt = b.copy_range("strideTT", 0, nc)
+    MapOT = b.copy_range("MapOT", 0, b.ot_len)
+    MapTT = b.copy_range("MapTT", 0, b.tt_len)
+    assert sot.max() < 2**31 and stt.max() < 2**31
+    rng = np.random.default_rng(1112)
+    C = rng.standard_normal(nc) + 1j * rng.standard_normal(nc)
+    C /= np.linalg.norm(C)
+    Ho, Hint = mcpi_amd.ll_ho(M), mcpi_amd.ll_hint(M, G_)
+    dC, dO = mcpi_amd.DeviceArray.from_numpy(C), mcpi_amd.DeviceArray(nc)
+    H = mcpi_amd.Hamiltonian(b, Ho, Hint, skip_zero=False, schedule=5)
+    H.apply(dC, dO)
+    gpu_ms = H.apply_timed(dC, dO, 20)
+    gpu_out = dO.numpy()
+    H.close()
+    times, rel = [], 0.0
+    while sum(times) < target_s and len(times) < 3 or not times:
+        t = time.perf_counter()
+        out = ref.acth_omp_tables(N, M, Map, MapOT, MapTT, sot, stt, IF, C, Ho, Hint)
+        times.append(time.perf_counter() - t)
+        rel = max(rel, float(np.linalg.norm(out - gpu_out) / np.linalg.norm(gpu_out)))
+    t_ref = float(np.median(times))
+    L = oracle.lib()
+    out = np.zeros(nc, np.complex128)
+    t = time.perf_counter()
+    L.orc_acth_range(N, M, 0, nc, Map, MapOT, MapTT, sot.astype(np.int64), stt.astype(np.int64), IF, C,
+                     oracle.flat(Ho), oracle.flat(Hint), out)
+    t_port = time.perf_counter() - t
+    rel_port = float(np.linalg.norm(out - gpu_out) / np.linalg.norm(gpu_out))
+    Bn = algorithmic_bytes(nc, M, b.ot_len, b.tt_len)
+    dC.free()
+    dO.free()
+    b.close()
+    return {"workload": "H.v N=11, M=12, Lieb-Liniger g=10, full maps (every term)", "nc": nc,
+            "bytes_per_hv": Bn, "threads": oracle.num_threads(), "cores": host_cores(),
+            "reference_ms_per_hv": round(t_ref * 1e3, 1), "reference_calls": len(times),
+            "reference_GB_s": round(Bn / t_ref / 1e9, 3),
+            "port_ms_per_hv": round(t_port * 1e3, 1), "gpu_ms_per_hv": round(gpu_ms, 4),
+            "gpu_vs_reference": round(t_ref * 1e3 / gpu_ms, 1),
+            "rel_diff_reference_vs_gpu": rel, "rel_diff_port_vs_gpu": rel_port}
 
 
 def batched_lanczos(n_inst: int = 64, lm: int = 120, seed: int = 64120, serial: bool = True,
@@ -226,20 +332,58 @@ def large_basis(reps: int = 20, lm: int = 200, quick: bool = False):
                       "hbm_frac_model": round(B / ms / 1e6 / HBM_PEAK_GBS, 4), "schedule": H.schedule,
                       "pattern_table_bytes": b.pattern_bytes()}
         H.close()
-    del dO
+    # the same H.v on the sector-ordered basis (rows sorted by total
+    # momentum, which the Lieb-Liniger Hamiltonian conserves): the kernel on
+    # row-order vectors (what every Lanczos step runs), and the reference-
+    # order entry point, which moves both vectors through the row order
+    t0 = time.time()
+    bs = mcpi_amd.Basis(N, M, "single", charge=mcpi_amd.conserved_charge(Ho, Hint))
+    sbuild_s = time.time() - t0
+    Hs = mcpi_amd.Hamiltonian(bs, Ho, Hint)
+    dR, dRo = mcpi_amd.DeviceArray(nc), mcpi_amd.DeviceArray(nc)
+    bs.to_rows(dC, dR)
+    Hs.apply_rows(dR, dRo)                   # pattern tables built here
+    ms_rows = Hs.apply_timed(dR, dRo, reps)
+    mcpi_amd.synchronize()
+    t = time.perf_counter()
+    for _ in range(reps):
+        Hs.apply(dC, dO)
+    mcpi_amd.synchronize()
+    ms_ref = (time.perf_counter() - t) / reps * 1e3
+    ref_out = None
+    if not quick:
+        Hp = mcpi_amd.Hamiltonian(b, Ho, Hint)
+        dP = mcpi_amd.DeviceArray(nc)
+        Hp.apply(dC, dP)
+        mcpi_amd.synchronize()
+        ref_out = dP.numpy()
+        Hp.close()
+        del dP
+    out["sector_order"] = {
+        "nsectors": bs.nsectors, "largest_sector_rows": bs.largest_sector, "basis_build_s": round(sbuild_s, 3),
+        "ms_per_hv": round(ms_rows, 4), "gb_s_model": round(B / ms_rows / 1e6, 1),
+        "hbm_frac_model": round(B / ms_rows / 1e6 / HBM_PEAK_GBS, 4), "schedule": Hs.schedule,
+        "pattern_table_bytes": bs.pattern_bytes(),
+        "ms_per_hv_reference_order_api": round(ms_ref, 4),
+        "max_rel_diff_vs_plain_basis": (float(np.abs(dO.numpy() - ref_out).max() / np.abs(ref_out).max())
+                                        if ref_out is not None else None)}
+    del dR, dRo
     if quick:
         return out
-    H = mcpi_amd.Hamiltonian(b, Ho, Hint)
     dS = mcpi_amd.DeviceArray.from_numpy(mcpi_amd.start_vector(nc))
-    t = time.perf_counter()
-    E0, psi, steps = H.ground(lm, dS)
-    t_ground = time.perf_counter() - t
-    t = time.perf_counter()
-    g1, g2 = b.vjp(psi)
-    t_vjp = time.perf_counter() - t
-    out["ground"] = {"lm": lm, "steps": steps, "E0_per_N": E0 / N, "s": round(t_ground, 3),
-                     "vjp_s": round(t_vjp, 3),
-                     "energy_identity_residual": float(abs(np.real(np.sum(g1 * Ho) + np.sum(g2 * Hint)) - E0))}
+    for label, basis, ham in (("ground", b, mcpi_amd.Hamiltonian(b, Ho, Hint)), ("ground_sector_order", bs, Hs)):
+        t = time.perf_counter()
+        E0, psi, steps = ham.ground(lm, dS)
+        t_ground = time.perf_counter() - t
+        t = time.perf_counter()
+        g1, g2 = basis.vjp(psi)
+        t_vjp = time.perf_counter() - t
+        out[label] = {"lm": lm, "steps": steps, "E0_per_N": E0 / N, "s": round(t_ground, 3),
+                      "vjp_s": round(t_vjp, 3),
+                      "energy_identity_residual": float(abs(np.real(np.sum(g1 * Ho) + np.sum(g2 * Hint)) - E0))}
+        del psi
+    out["ground_sector_order"]["abs_dE0_vs_plain"] = abs(out["ground_sector_order"]["E0_per_N"]
+                                                         - out["ground"]["E0_per_N"]) * N
     return out
 
 
@@ -370,9 +514,11 @@ def main() -> int:
     cpu = None
     if not (args.no_cpu or args.profile):
         cpu = cpu_baseline(b, C, Ho, Hint, gpu_out, args.cpu_seconds, gpu_rho)
+        cpu["reference_build"] = reference_build_baseline()
     if not (args.no_large or args.profile):
         large = large_basis()
 
+    traffic, traffic_info = traffic_from_profiles()
     line = {
         "metric": "H·v GB/s (% HBM peak)",
         "value": round(value, 3),
@@ -399,11 +545,11 @@ def main() -> int:
             "peak": HBM_PEAK_GBS,
             "unit": "GB/s",
             "frac": round(achieved / HBM_PEAK_GBS, 4),
-            "traffic": traffic_from_profiles(),
+            "traffic": traffic,
         },
         "cpu_baseline": cpu,
         "extra": {
-            "device": name, "compute_units": cus,
+            "device": name, "compute_units": cus, "traffic_record": traffic_info,
             "hbm_fraction_wall": round(value / HBM_PEAK_GBS, 4),
             "kernel_ms_per_hv": round(kern_ms, 4),
             "kernel_ms_p10_p50_p90": [round(float(np.percentile(each, q)), 4) for q in (10, 50, 90)],

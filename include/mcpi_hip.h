@@ -171,6 +171,9 @@ typedef struct {
                                  layout is given at mcpi_hip_basis_copy_range) */
     int64_t pattern_bytes;    /* device memory of the pattern tables built so
                                  far for H.v schedule 4 (all zero patterns) */
+    int64_t nsectors;         /* charge sectors of a sector-ordered basis (0:
+                                 reference order) */
+    int64_t largest_sector;   /* rows of its largest sector */
 } mcpi_basis_info;
 
 const char *mcpi_hip_last_error(void);
@@ -198,15 +201,20 @@ mcpi_status mcpi_hip_device_info(char *name, size_t name_len, int *cu_count,
 mcpi_status mcpi_hip_basis_create(int N, int M, int maps, mcpi_basis *out);
 mcpi_status mcpi_hip_basis_destroy(mcpi_basis b);
 mcpi_status mcpi_hip_basis_info(mcpi_basis b, mcpi_basis_info *info);
-/* copies of the tables to host memory (sizes from mcpi_basis_info) */
+/* copies of the tables to host memory (sizes from mcpi_basis_info); IF in the
+ * reference order (also for a sector-ordered basis, whose map is in row order
+ * and is copied with mcpi_hip_basis_copy_range only) */
 mcpi_status mcpi_hip_basis_copy_ncmat(mcpi_basis b, int32_t *NCmat);
 mcpi_status mcpi_hip_basis_copy_if(mcpi_basis b, int32_t *IF);
 mcpi_status mcpi_hip_basis_copy_map(mcpi_basis b, int32_t *Map);
 mcpi_status mcpi_hip_basis_copy_ot(mcpi_basis b, int64_t *strideOT, int32_t *MapOT);
 mcpi_status mcpi_hip_basis_copy_tt(mcpi_basis b, int64_t *strideTT, int32_t *MapTT);
 /* count entries of one table from offset (which: 0 Map, 1 MapOT, 2 MapTT,
- * 3 IF, 4 strideOT, 5 strideTT, 6 packOT, 7 packTT, 8 poffOT, 9 poffTT;
- * strides and packed offsets are int64, the others int32).
+ * 3 IF, 4 strideOT, 5 strideTT, 6 packOT, 7 packTT, 8 poffOT, 9 poffTT,
+ * and for sector-ordered bases 10 perm (reference index -> row), 11 iperm
+ * (row -> reference index), 12 IF in the reference order; strides and
+ * packed offsets are int64, the others int32).  Tables are in the basis's
+ * row order and hold row indices.
  * Wave-packed tables (full maps): wavefront w = rows 64w .. 64w+63; for each
  * jump active in at least one of its rows (OT: k with v_k >= 2, ascending;
  * TT: pairs k < s both occupied, lexicographic) one slot of E entries per
@@ -220,6 +228,35 @@ mcpi_status mcpi_hip_basis_copy_tt(mcpi_basis b, int64_t *strideTT, int32_t *Map
  * target of (q, l) is the reference map's entry l + q*M of that chunk. */
 mcpi_status mcpi_hip_basis_copy_range(mcpi_basis b, int which, int64_t offset,
                                       int64_t count, void *dst);
+/* Sector order.  A Hamiltonian that conserves an additive charge
+ * Q(v) = sum_j q_j v_j (Ho[k,l] != 0 only where q_k = q_l, Hint[k,s,q,l] != 0
+ * only where q_k + q_s = q_q + q_l) is block-diagonal over the sectors of
+ * equal Q; the Lieb-Liniger model conserves the total momentum, q_j = j
+ * (groundStateLieb.c:143-153, 200-207).  A sector-ordered basis stores its
+ * rows sorted by Q (reference order kept within a sector), so the H.v of a
+ * sector gathers only from that sector's slice of the vector.  Every entry
+ * point that takes or returns vectors (mcpi_hip_applyH, the Lanczos and
+ * ground-state solvers, the densities) keeps the reference order: the
+ * solvers move the start vector into row order once and the Ritz vector
+ * back once; mcpi_hip_applyH moves both vectors on every call, and
+ * mcpi_hip_applyH_rows / mcpi_hip_applyH_timed* work on row-order vectors.
+ * Only the single-jump map is stored (maps must be MCPI_MAPS_SINGLE) and
+ * H.v runs the zero-skipping kernel on the pattern tables (schedule 4);
+ * 3 <= M <= 13.  Hamiltonians that couple two sectors are refused. */
+/* charge[j] = j if Ho and Hint conserve the orbital-index sum (MCPI_OK),
+ * MCPI_ERR_STATE otherwise */
+mcpi_status mcpi_hip_conserved_charge(int M, const mcpi_cplx *Ho, const mcpi_cplx *Hint,
+                                      int32_t *charge);
+mcpi_status mcpi_hip_basis_create_sectored(int N, int M, int maps, const int32_t *charge,
+                                           mcpi_basis *out);
+/* number of sectors and their row offsets (offsets: nsectors + 1 entries,
+ * may be NULL); a basis in the reference order reports one sector */
+mcpi_status mcpi_hip_basis_sectors(mcpi_basis b, int64_t *nsectors, int64_t *offsets);
+/* vector (nc complex, device) from the reference order into row order
+ * (to_rows = 1) or back (0), on stream; a plain copy for other bases */
+mcpi_status mcpi_hip_basis_reorder(mcpi_basis b, int to_rows, const mcpi_cplx *in_dev,
+                                   mcpi_cplx *out_dev, void *stream);
+
 /* rank / unrank of many states on the GPU: v is n*M ints (host) */
 mcpi_status mcpi_hip_basis_rank(mcpi_basis b, int64_t n, const int32_t *v, int64_t *idx);
 mcpi_status mcpi_hip_basis_unrank(mcpi_basis b, int64_t n, const int64_t *idx, int32_t *v);
@@ -252,6 +289,10 @@ mcpi_status mcpi_hip_ham_get_schedule(mcpi_ham h, int *schedule);
 /* out = H C on device pointers (nc complex each), on stream (NULL = default) */
 mcpi_status mcpi_hip_applyH(mcpi_ham h, const mcpi_cplx *C_dev, mcpi_cplx *out_dev,
                             void *stream);
+/* out = H C with both vectors in the basis's row order (the reference order
+ * except for sector-ordered bases), on stream */
+mcpi_status mcpi_hip_applyH_rows(mcpi_ham h, const mcpi_cplx *C_dev, mcpi_cplx *out_dev,
+                                 void *stream);
 /* CUDA-kernel-shaped H.v on raw device tables (cuda/hamiltonianMatrix.cuh:61):
  * every pointer is device memory, Ho flat k + M*l, int64 strides (nc
  * entries).  Every term is evaluated.  Enqueued on stream (NULL = default)
@@ -271,8 +312,8 @@ mcpi_status mcpi_hip_applyHconf_dev32(int N, int M, const int32_t *Map,
                                       const int32_t *IF, const mcpi_cplx *C,
                                       const mcpi_cplx *Ho, const mcpi_cplx *Hint,
                                       mcpi_cplx *out, void *stream);
-/* average duration (ms) of the H.v kernels of the last n calls of
- * mcpi_hip_applyH_timed, measured with HIP events on the launch stream */
+/* average duration (ms) of reps H.v kernels (mcpi_hip_applyH_rows),
+ * measured with HIP events on the launch stream */
 mcpi_status mcpi_hip_applyH_timed(mcpi_ham h, const mcpi_cplx *C_dev,
                                   mcpi_cplx *out_dev, int reps, void *stream,
                                   double *avg_ms);

@@ -51,7 +51,8 @@ class _BasisInfo(ct.Structure):
     _fields_ = [("N", ct.c_int), ("M", ct.c_int), ("maps", ct.c_int),
                 ("nc", ct.c_int64), ("map_len", ct.c_int64), ("ot_len", ct.c_int64),
                 ("tt_len", ct.c_int64), ("device_bytes", ct.c_int64),
-                ("pot_len", ct.c_int64), ("ptt_len", ct.c_int64), ("pattern_bytes", ct.c_int64)]
+                ("pot_len", ct.c_int64), ("ptt_len", ct.c_int64), ("pattern_bytes", ct.c_int64),
+                ("nsectors", ct.c_int64), ("largest_sector", ct.c_int64)]
 
 
 _lib = None
@@ -86,6 +87,11 @@ def lib():
         "mcpi_hip_device_info": (st, [ct.c_char_p, ct.c_size_t, ct.POINTER(ct.c_int),
                                       ct.POINTER(ct.c_size_t)]),
         "mcpi_hip_basis_create": (st, [ct.c_int, ct.c_int, ct.c_int, ct.POINTER(_vp)]),
+        "mcpi_hip_basis_create_sectored": (st, [ct.c_int, ct.c_int, ct.c_int, _i32p, ct.POINTER(_vp)]),
+        "mcpi_hip_basis_sectors": (st, [_vp, ct.POINTER(ct.c_int64), _vp]),
+        "mcpi_hip_basis_reorder": (st, [_vp, ct.c_int, _vp, _vp, _vp]),
+        "mcpi_hip_conserved_charge": (st, [ct.c_int, _c128p, _c128p, _i32p]),
+        "mcpi_hip_applyH_rows": (st, [_vp, _vp, _vp, _vp]),
         "mcpi_hip_basis_destroy": (st, [_vp]),
         "mcpi_hip_basis_info": (st, [_vp, ct.POINTER(_BasisInfo)]),
         "mcpi_hip_basis_copy_ncmat": (st, [_vp, _i32p]),
@@ -232,15 +238,32 @@ def device_info():
 
 # ------------------------------------------------------------- basis --
 
+def conserved_charge(Ho, Hint):
+    """Per-orbital charges q_j that Ho and Hint conserve (q_j = j when they
+    conserve the orbital-index sum, e.g. Lieb-Liniger momentum), or None."""
+    M = np.asarray(Ho).shape[0]
+    q = np.zeros(M, np.int32)
+    st = lib().mcpi_hip_conserved_charge(M, flat(Ho), flat(Hint), q)
+    return q if st == 0 else None
+
+
 class Basis:
     """Device-resident basis of N bosons in M orbitals (tables built on the GPU).
 
     ``maps``: "full" (Map, MapOT, MapTT), "single" (Map; double jumps ranked
-    on the fly) or "none" (every jump ranked on the fly)."""
+    on the fly) or "none" (every jump ranked on the fly).  ``charge``
+    (per-orbital integers): rows sorted by the conserved charge
+    sum_j q_j v_j (sector order; maps must be "single").  Vectors passed to
+    the methods are always in the reference order."""
 
-    def __init__(self, N: int, M: int, maps: str = "full"):
+    def __init__(self, N: int, M: int, maps: str = "full", charge=None):
         h = _vp()
-        _check(lib().mcpi_hip_basis_create(N, M, _MAPS[maps], ct.byref(h)), f"basis({N},{M})")
+        if charge is None:
+            _check(lib().mcpi_hip_basis_create(N, M, _MAPS[maps], ct.byref(h)), f"basis({N},{M})")
+        else:
+            q = np.ascontiguousarray(charge, np.int32)
+            _check(lib().mcpi_hip_basis_create_sectored(N, M, _MAPS[maps], q, ct.byref(h)),
+                   f"sectored basis({N},{M})")
         self.handle = h.value
         _live.add(self)
         info = _BasisInfo()
@@ -250,6 +273,26 @@ class Basis:
         self.map_len, self.ot_len, self.tt_len = int(info.map_len), int(info.ot_len), int(info.tt_len)
         self.device_bytes = int(info.device_bytes)
         self.pot_len, self.ptt_len = int(info.pot_len), int(info.ptt_len)
+        self.nsectors, self.largest_sector = int(info.nsectors), int(info.largest_sector)
+
+    @property
+    def sectored(self) -> bool:
+        return self.nsectors > 0
+
+    def sector_offsets(self) -> np.ndarray:
+        n = ct.c_int64(0)
+        _check(lib().mcpi_hip_basis_sectors(self.handle, ct.byref(n), None), "sectors")
+        off = np.zeros(n.value + 1, np.int64)
+        _check(lib().mcpi_hip_basis_sectors(self.handle, ct.byref(n), off.ctypes.data), "sectors")
+        return off
+
+    def to_rows(self, src: "DeviceArray", dst: "DeviceArray", stream=None) -> None:
+        """dst = src moved from the reference order into the basis's row order."""
+        _check(lib().mcpi_hip_basis_reorder(self.handle, 1, src.ptr, dst.ptr, stream), "reorder")
+
+    def from_rows(self, src: "DeviceArray", dst: "DeviceArray", stream=None) -> None:
+        """dst = src moved from the basis's row order into the reference order."""
+        _check(lib().mcpi_hip_basis_reorder(self.handle, 0, src.ptr, dst.ptr, stream), "reorder")
 
     def ncmat(self) -> np.ndarray:
         out = np.zeros((self.N + 1) * (self.M + 1), np.int32)
@@ -281,7 +324,8 @@ class Basis:
     _WHICH = {"Map": (0, np.int32), "MapOT": (1, np.int32), "MapTT": (2, np.int32),
               "IF": (3, np.int32), "strideOT": (4, np.int64), "strideTT": (5, np.int64),
               "packOT": (6, np.int32), "packTT": (7, np.int32), "poffOT": (8, np.int64),
-              "poffTT": (9, np.int64)}
+              "poffTT": (9, np.int64), "perm": (10, np.int32), "iperm": (11, np.int32),
+              "IF_ref": (12, np.int32)}
 
     def pattern_bytes(self) -> int:
         """Device memory of the pattern tables built so far (schedule 4)."""
@@ -372,6 +416,10 @@ class Hamiltonian:
 
     def apply(self, C: DeviceArray, out: DeviceArray, stream=None) -> None:
         _check(lib().mcpi_hip_applyH(self.handle, C.ptr, out.ptr, stream), "applyH")
+
+    def apply_rows(self, C: DeviceArray, out: DeviceArray, stream=None) -> None:
+        """H.v with both vectors in the basis's row order."""
+        _check(lib().mcpi_hip_applyH_rows(self.handle, C.ptr, out.ptr, stream), "applyH_rows")
 
     def apply_timed(self, C: DeviceArray, out: DeviceArray, reps: int, stream=None) -> float:
         ms = ct.c_double(0)

@@ -462,6 +462,25 @@ static bool pattern_schedule_ok(mcpi_ham h)
 static mcpi_status launch_sched(mcpi_ham h, int sched, const cplx *C, cplx *out, hipStream_t st)
 {
     const BasisDev &b = h->basis->d;
+    if (h->basis->sectored()) {
+        // rows in sector order: only the pattern tables (built with the
+        // basis's row indices) give the targets; every other schedule ranks
+        // or reads reference-order tables
+        hipStreamCaptureStatus cs = hipStreamCaptureStatusNone;
+        MCPI_HIP(hipStreamIsCapturing(st, &cs));
+        const PatternTables *pt = basis_pattern(h->basis, h->c, cs == hipStreamCaptureStatusNone, st);
+        if (!pt) {
+            set_error("sector-ordered basis: the pattern tables of this Hamiltonian could not be built "
+                      "(device memory short, or first H.v under stream capture)");
+            return MCPI_ERR_STATE;
+        }
+        BasisDev bd = b;
+        bd.patOT = pt->tabOT;
+        bd.patTT = pt->tabTT;
+        bd.ppoffOT = pt->offOT;
+        bd.ppoffTT = pt->offTT;
+        return launch_apply_raw(bd, h->basis->maps, h->c, 1, 2, C, out, st);
+    }
     if (sched == 4) {
         const PatternTables *pt = nullptr;
         if (pattern_schedule_ok(h)) {
@@ -486,6 +505,10 @@ static mcpi_status autotune(mcpi_ham h, const cplx *C, cplx *out, hipStream_t st
 {
     const BasisDev &b = h->basis->d;
     const int maps = h->basis->maps;
+    if (h->basis->sectored()) {   // one schedule applies
+        h->tuned[h->skip_zero ? 1 : 0] = 4;
+        return MCPI_OK;
+    }
     int cands[5], n = 0;
     cands[n++] = 1;
     if (has_fast_kernel(b.N, b.M)) {

@@ -124,7 +124,13 @@ __global__ void k_one_one_map(mcpi::BasisDev b)
         for (int l = 0; l < M; l++) {
             int64_t at = i + (int64_t)(k + l * M) * b.nc;
             int32_t val = -1;
-            if (occ) val = (l == k) ? (int32_t)i : (int32_t)rank_moved(Ploc, Bt, N, M, b.nc, k, -1, l, -1);
+            if (occ) {
+                if (l == k) val = (int32_t)i;
+                else {
+                    const int64_t j = rank_moved(Ploc, Bt, N, M, b.nc, k, -1, l, -1);
+                    val = b.perm ? b.perm[j] : (int32_t)j;
+                }
+            }
             b.Map[at] = val;
         }
     }
@@ -454,11 +460,60 @@ __global__ void k_pattern_fill(mcpi::BasisDev b, HamCoefDev c, const int64_t *__
             if (act)
                 for (int t = 0; t < n; t++) {
                     const int e = c.nz_e[beg + t];
-                    tab[base + (int64_t)t * nact + my] = (int32_t)rank_moved(P, Bt, N, M, b.nc, k, s, e / M, e % M);
+                    const int64_t j = rank_moved(P, Bt, N, M, b.nc, k, s, e / M, e % M);
+                    tab[base + (int64_t)t * nact + my] = b.perm ? b.perm[j] : (int32_t)j;
                 }
             base += (int64_t)nact * n;
         }
     }
+}
+
+// ---- sector order --
+// charge of every state: Q(v) = sum_j q_j v_j
+__global__ void k_state_charge(int M, int64_t nc, const int32_t *__restrict__ IF, const int32_t *__restrict__ q,
+                               int32_t *__restrict__ out)
+{
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= nc) return;
+    int32_t s = 0;
+    for (int j = 0; j < M; j++) s += q[j] * IF[i * M + j];
+    out[i] = s;
+}
+
+__global__ void k_gather_rows(int64_t n, const int32_t *__restrict__ idx, const cplx *__restrict__ in,
+                              cplx *__restrict__ out)
+{
+    for (int64_t r = (int64_t)blockIdx.x * 256 + threadIdx.x; r < n; r += (int64_t)gridDim.x * 256)
+        out[r] = in[idx[r]];
+}
+
+__global__ void k_scatter_rows(int64_t n, const int32_t *__restrict__ idx, const cplx *__restrict__ in,
+                               cplx *__restrict__ out)
+{
+    for (int64_t r = (int64_t)blockIdx.x * 256 + threadIdx.x; r < n; r += (int64_t)gridDim.x * 256)
+        out[idx[r]] = in[r];
+}
+
+__global__ void k_widen_rows(int64_t n, const int32_t *__restrict__ a, int64_t *__restrict__ w)
+{
+    const int64_t r = (int64_t)blockIdx.x * 256 + threadIdx.x;
+    if (r < n) w[r] = a[r];
+}
+
+static unsigned row_blocks(int64_t n) { return (unsigned)std::min<int64_t>(std::max<int64_t>((n + 255) / 256, 1), 8192); }
+
+mcpi_status gather_rows(int64_t n, const int32_t *idx, const cplx *in, cplx *out, hipStream_t st)
+{
+    hipLaunchKernelGGL(k_gather_rows, dim3(row_blocks(n)), dim3(256), 0, st, n, idx, in, out);
+    MCPI_HIP(hipGetLastError());
+    return MCPI_OK;
+}
+
+mcpi_status scatter_rows(int64_t n, const int32_t *idx, const cplx *in, cplx *out, hipStream_t st)
+{
+    hipLaunchKernelGGL(k_scatter_rows, dim3(row_blocks(n)), dim3(256), 0, st, n, idx, in, out);
+    MCPI_HIP(hipGetLastError());
+    return MCPI_OK;
 }
 
 }  // namespace mcpi
@@ -515,6 +570,9 @@ static void free_basis(mcpi_basis b)
             if (p) (void)hipFree(p);
     }
     if (b->rho_ws) (void)hipFree(b->rho_ws);
+    void *sec[] = {(void *)b->d.perm, b->iperm, b->IF_ref, b->vt[0], b->vt[1]};
+    for (void *p : sec)
+        if (p) (void)hipFree(p);
     delete b;
 }
 
@@ -662,7 +720,75 @@ static mcpi_status basis_build_maps(mcpi_basis b, int maps)
     return build_maps_parts(b, maps >= MCPI_MAPS_SINGLE, maps >= MCPI_MAPS_FULL, maps >= MCPI_MAPS_FULL);
 }
 
+// Sector order: rows sorted by the conserved charge Q(v) = sum_j q_j v_j,
+// the reference order kept inside a sector (stable counting sort on the
+// host over the charges computed on the device).  IF is rebuilt in row
+// order by unranking the reference indices, and the single-jump map holds
+// row indices.  A Hamiltonian that conserves Q only couples rows of one
+// sector, so the H.v of a sector gathers from that sector's slice of the
+// vector.
+static mcpi_status make_sectored(mcpi_basis b, const int32_t *charge)
+{
+    const int M = b->d.M, N = b->d.N;
+    const int64_t nc = b->d.nc;
+    b->charge.assign(charge, charge + M);
+    int32_t *dq = nullptr, *dQ = nullptr;
+    MCPI_HIP(hipMalloc(&dq, sizeof(int32_t) * M));
+    MCPI_HIP(hipMalloc(&dQ, sizeof(int32_t) * nc));
+    MCPI_HIP(hipMemcpy(dq, charge, sizeof(int32_t) * M, hipMemcpyHostToDevice));
+    hipLaunchKernelGGL(k_state_charge, dim3((unsigned)((nc + 255) / 256)), dim3(256), 0, 0, M, nc, b->d.IF, dq, dQ);
+    MCPI_HIP(hipGetLastError());
+    std::vector<int32_t> Q(nc);
+    MCPI_HIP(hipMemcpy(Q.data(), dQ, sizeof(int32_t) * nc, hipMemcpyDeviceToHost));
+    MCPI_HIP(hipFree(dq));
+    const int32_t qmin = *std::min_element(Q.begin(), Q.end()), qmax = *std::max_element(Q.begin(), Q.end());
+    std::vector<int64_t> start((size_t)(qmax - qmin) + 2, 0);
+    for (int64_t i = 0; i < nc; i++) start[(size_t)(Q[i] - qmin) + 1]++;
+    b->sec_off.assign(1, 0);
+    for (size_t c = 1; c < start.size(); c++) {
+        if (start[c]) b->sec_off.push_back(b->sec_off.back() + start[c]);
+        start[c] += start[c - 1];
+    }
+    std::vector<int32_t> iperm(nc), perm(nc);
+    for (int64_t i = 0; i < nc; i++) {
+        const int64_t r = start[(size_t)(Q[i] - qmin)]++;
+        iperm[r] = (int32_t)i;
+        perm[i] = (int32_t)r;
+    }
+    int32_t *dperm = nullptr;
+    MCPI_CHECK(alloc_dev((void **)&dperm, nc * 4, &b->device_bytes));
+    b->d.perm = dperm;
+    MCPI_CHECK(alloc_dev((void **)&b->iperm, nc * 4, &b->device_bytes));
+    MCPI_HIP(hipMemcpy(dperm, perm.data(), sizeof(int32_t) * nc, hipMemcpyHostToDevice));
+    MCPI_HIP(hipMemcpy(b->iperm, iperm.data(), sizeof(int32_t) * nc, hipMemcpyHostToDevice));
+    // IF in row order: unrank the reference index of every row (the
+    // reference-order IF is kept for the density matrices and for copies)
+    MCPI_HIP(hipFree(dQ));
+    int64_t *idx64 = nullptr;
+    MCPI_HIP(hipMalloc(&idx64, sizeof(int64_t) * nc));
+    hipLaunchKernelGGL(k_widen_rows, dim3((unsigned)((nc + 255) / 256)), dim3(256), 0, 0, nc, b->iperm, idx64);
+    int32_t *IFrow = nullptr;
+    MCPI_CHECK(alloc_dev((void **)&IFrow, nc * M * 4, &b->device_bytes));
+    hipLaunchKernelGGL(k_unrank_if, dim3((unsigned)((nc + 255) / 256)), dim3(256), 0, 0, N, M, nc, b->d.NCmat,
+                       (const int64_t *)idx64, IFrow);
+    MCPI_HIP(hipGetLastError());
+    MCPI_HIP(hipDeviceSynchronize());
+    MCPI_HIP(hipFree(idx64));
+    b->IF_ref = b->d.IF;
+    b->d.IF = IFrow;
+    return MCPI_OK;
+}
+
 namespace mcpi {
+
+mcpi_status basis_vectors(mcpi_basis b, cplx **v0, cplx **v1)
+{
+    for (int k = 0; k < 2; k++)
+        if (!b->vt[k]) MCPI_CHECK(alloc_dev((void **)&b->vt[k], sizeof(cplx) * b->d.nc, &b->device_bytes));
+    *v0 = b->vt[0];
+    if (v1) *v1 = b->vt[1];
+    return MCPI_OK;
+}
 
 // Pattern tables are kept for at most this many zero patterns per basis
 // (least recently used dropped first) and only while they leave this much
@@ -872,6 +998,58 @@ mcpi_status mcpi_hip_basis_create(int N, int M, int maps, mcpi_basis *out)
     return MCPI_OK;
 }
 
+mcpi_status mcpi_hip_basis_create_sectored(int N, int M, int maps, const int32_t *charge, mcpi_basis *out)
+{
+    MCPI_ARG(out && charge, "null argument");
+    MCPI_ARG(maps == MCPI_MAPS_SINGLE,
+             "a sector-ordered basis stores the single-jump map only (maps must be MCPI_MAPS_SINGLE)");
+    MCPI_ARG(M >= 1 && M <= MCPI_MAXM, "M out of range (1..32)");
+    MCPI_ARG(has_fast_kernel(N, M), "a sector-ordered basis needs 3 <= M <= 13 (its H.v kernel is specialised on M)");
+    int64_t span = 0;
+    for (int j = 0; j < M; j++) {
+        MCPI_ARG(charge[j] >= -65536 && charge[j] <= 65536, "charges must lie in [-65536, 65536]");
+        span = std::max<int64_t>(span, std::abs((int64_t)charge[j]));
+    }
+    MCPI_ARG(span * N <= INT32_MAX / 2, "charge times N out of int32 range");
+    mcpi_basis b = nullptr;
+    MCPI_CHECK(basis_init(N, M, nullptr, nullptr, &b));
+    mcpi_status st = make_sectored(b, charge);
+    if (st == MCPI_OK) st = basis_build_maps(b, maps);
+    if (st != MCPI_OK) {
+        free_basis(b);
+        return st;
+    }
+    *out = b;
+    return MCPI_OK;
+}
+
+mcpi_status mcpi_hip_basis_sectors(mcpi_basis b, int64_t *nsectors, int64_t *offsets)
+{
+    MCPI_ARG(b && nsectors, "null argument");
+    if (!b->sectored()) {
+        *nsectors = 1;
+        if (offsets) { offsets[0] = 0; offsets[1] = b->d.nc; }
+        return MCPI_OK;
+    }
+    *nsectors = (int64_t)b->sec_off.size() - 1;
+    if (offsets) std::copy(b->sec_off.begin(), b->sec_off.end(), offsets);
+    return MCPI_OK;
+}
+
+mcpi_status mcpi_hip_basis_reorder(mcpi_basis b, int to_rows, const mcpi_cplx *in_dev, mcpi_cplx *out_dev,
+                                   void *stream)
+{
+    MCPI_ARG(b && in_dev && out_dev, "null argument");
+    MCPI_ARG((const void *)in_dev != (const void *)out_dev, "input and output must not alias");
+    const hipStream_t st = (hipStream_t)stream;
+    if (!b->sectored()) {
+        MCPI_HIP(hipMemcpyAsync(out_dev, in_dev, sizeof(cplx) * b->d.nc, hipMemcpyDeviceToDevice, st));
+        return MCPI_OK;
+    }
+    if (to_rows) return gather_rows(b->d.nc, b->iperm, (const cplx *)in_dev, (cplx *)out_dev, st);
+    return scatter_rows(b->d.nc, b->iperm, (const cplx *)in_dev, (cplx *)out_dev, st);
+}
+
 mcpi_status mcpi_hip_basis_destroy(mcpi_basis b)
 {
     if (b) {
@@ -896,6 +1074,10 @@ mcpi_status mcpi_hip_basis_info(mcpi_basis b, mcpi_basis_info *info)
     info->ptt_len = b->ptt_len;
     info->pattern_bytes = 0;
     for (const PatternTables &t : b->patterns) info->pattern_bytes += t.bytes;
+    info->nsectors = b->sectored() ? (int64_t)b->sec_off.size() - 1 : 0;
+    info->largest_sector = 0;
+    for (size_t x = 0; x + 1 < b->sec_off.size(); x++)
+        info->largest_sector = std::max(info->largest_sector, b->sec_off[x + 1] - b->sec_off[x]);
     return MCPI_OK;
 }
 
@@ -910,7 +1092,8 @@ mcpi_status mcpi_hip_basis_copy_ncmat(mcpi_basis b, int32_t *NCmat)
 mcpi_status mcpi_hip_basis_copy_if(mcpi_basis b, int32_t *IF)
 {
     MCPI_ARG(b && IF, "null argument");
-    MCPI_HIP(hipMemcpy(IF, b->d.IF, sizeof(int32_t) * b->d.nc * b->d.M, hipMemcpyDeviceToHost));
+    const int32_t *src = b->sectored() ? b->IF_ref : b->d.IF;   // reference order
+    MCPI_HIP(hipMemcpy(IF, src, sizeof(int32_t) * b->d.nc * b->d.M, hipMemcpyDeviceToHost));
     return MCPI_OK;
 }
 
@@ -918,6 +1101,8 @@ mcpi_status mcpi_hip_basis_copy_map(mcpi_basis b, int32_t *Map)
 {
     MCPI_ARG(b && Map, "null argument");
     MCPI_ARG(b->d.Map, "basis has no single-jump map");
+    MCPI_ARG(!b->sectored(), "the single-jump map of a sector-ordered basis is in row order: "
+                             "copy it with mcpi_hip_basis_copy_range (which 0)");
     MCPI_HIP(hipMemcpy(Map, b->d.Map, sizeof(int32_t) * b->map_len, hipMemcpyDeviceToHost));
     return MCPI_OK;
 }
@@ -960,7 +1145,10 @@ mcpi_status mcpi_hip_basis_copy_range(mcpi_basis b, int which, int64_t offset, i
     case 7: src = b->d.packTT; len = b->ptt_len; break;
     case 8: src = b->d.poffOT; len = b->d.poffOT ? (b->d.nc + 63) / 64 + 1 : 0; esize = 8; break;
     case 9: src = b->d.poffTT; len = b->d.poffTT ? (b->d.nc + 63) / 64 + 1 : 0; esize = 8; break;
-    default: MCPI_ARG(false, "which must be 0..9");
+    case 10: src = b->d.perm; len = b->d.perm ? b->d.nc : 0; break;
+    case 11: src = b->iperm; len = b->iperm ? b->d.nc : 0; break;
+    case 12: src = b->IF_ref; len = b->IF_ref ? b->d.nc * b->d.M : 0; break;
+    default: MCPI_ARG(false, "which must be 0..12");
     }
     MCPI_ARG(src, "table not stored in this basis");
     MCPI_ARG(offset + count <= len, "range beyond the table");

@@ -130,8 +130,31 @@ mcpi_status mcpi_hip_ham_create(mcpi_basis b, const mcpi_cplx *Ho, const mcpi_cp
     h->schedule = 0;
     h->tuned[0] = h->tuned[1] = 0;
     h->Ho_dev = h->Hint_dev = nullptr;
+    if (b->sectored()) {
+        // the Hamiltonian must not couple two sectors
+        const int32_t *q = b->charge.data();
+        const cplx *ho = (const cplx *)Ho, *hi = (const cplx *)Hint;
+        bool ok = true;
+        for (int k = 0; k < M && ok; k++)
+            for (int l = 0; l < M && ok; l++) {
+                const cplx z = ho[k + (size_t)M * l];
+                ok = (z.re == 0 && z.im == 0) || q[k] == q[l];
+            }
+        for (size_t x = 0; x < (size_t)M * M * M * M && ok; x++) {
+            const cplx z = hi[x];
+            const int a = (int)(x % M), c = (int)(x / M % M), e = (int)(x / M / M % M), f = (int)(x / M / M / M);
+            ok = (z.re == 0 && z.im == 0) || q[a] + q[c] == q[e] + q[f];
+        }
+        if (!ok) {
+            delete h;
+            MCPI_ARG(false, "the Hamiltonian does not conserve the charge of this sector-ordered basis");
+        }
+    }
     mcpi_status st = build_coefficients(M, (const cplx *)Ho, (const cplx *)Hint, &h->c, &h->coef_block,
                                         &h->coef_bytes);
+    // sector order: the zero-skipping kernel on the pattern tables is the one
+    // H.v, so its term lists are walked even when most terms are non-zero
+    if (st == MCPI_OK && b->sectored()) h->c.dense = 0;
     if (st == MCPI_OK) {
         const size_t nHo = (size_t)M * M, nHi = nHo * nHo;
         hipError_t e = hipMalloc(&h->Ho_dev, sizeof(cplx) * nHo);
@@ -162,6 +185,7 @@ mcpi_status mcpi_hip_ham_destroy(mcpi_ham h)
 mcpi_status mcpi_hip_ham_set_skip_zero(mcpi_ham h, int on)
 {
     MCPI_ARG(h, "null handle");
+    MCPI_ARG(on || !h->basis->sectored(), "a sector-ordered basis evaluates the non-zero terms only");
     h->skip_zero = on ? 1 : 0;
     return MCPI_OK;
 }
@@ -170,6 +194,8 @@ mcpi_status mcpi_hip_ham_set_schedule(mcpi_ham h, int schedule)
 {
     MCPI_ARG(h, "null handle");
     MCPI_ARG(schedule >= 0 && schedule <= 5, "schedule must be 0 (auto), 1, 2, 3, 4 or 5");
+    MCPI_ARG(schedule == 0 || schedule == 4 || !h->basis->sectored(),
+             "a sector-ordered basis runs schedule 4 (pattern tables) only");
     h->schedule = schedule;
     return MCPI_OK;
 }
@@ -185,7 +211,46 @@ mcpi_status mcpi_hip_applyH(mcpi_ham h, const mcpi_cplx *C_dev, mcpi_cplx *out_d
 {
     MCPI_ARG(h && C_dev && out_dev, "null argument");
     MCPI_ARG((const void *)C_dev != (const void *)out_dev, "input and output must not alias");
+    const hipStream_t st = (hipStream_t)stream;
+    mcpi_basis b = h->basis;
+    if (!b->sectored()) return launch_apply(h, (const cplx *)C_dev, (cplx *)out_dev, st);
+    // reference order in and out: gather into row order, H.v, scatter back
+    cplx *v0 = nullptr, *v1 = nullptr;
+    MCPI_CHECK(basis_vectors(b, &v0, &v1));
+    MCPI_CHECK(gather_rows(b->d.nc, b->iperm, (const cplx *)C_dev, v0, st));
+    MCPI_CHECK(launch_apply(h, v0, v1, st));
+    return scatter_rows(b->d.nc, b->iperm, v1, (cplx *)out_dev, st);
+}
+
+mcpi_status mcpi_hip_applyH_rows(mcpi_ham h, const mcpi_cplx *C_dev, mcpi_cplx *out_dev, void *stream)
+{
+    MCPI_ARG(h && C_dev && out_dev, "null argument");
+    MCPI_ARG((const void *)C_dev != (const void *)out_dev, "input and output must not alias");
     return launch_apply(h, (const cplx *)C_dev, (cplx *)out_dev, (hipStream_t)stream);
+}
+
+mcpi_status mcpi_hip_conserved_charge(int M, const mcpi_cplx *Ho, const mcpi_cplx *Hint, int32_t *charge)
+{
+    MCPI_ARG(M >= 1 && M <= MCPI_MAXM && Ho && Hint && charge, "bad argument");
+    // candidate: the orbital index (plane-wave momentum up to a shift)
+    const cplx *ho = (const cplx *)Ho, *hi = (const cplx *)Hint;
+    bool ok = true;
+    for (int k = 0; k < M && ok; k++)
+        for (int l = 0; l < M && ok; l++) {
+            const cplx z = ho[k + (size_t)M * l];
+            ok = (z.re == 0 && z.im == 0) || k == l;
+        }
+    for (size_t x = 0; x < (size_t)M * M * M * M && ok; x++) {
+        const cplx z = hi[x];
+        const int a = (int)(x % M), c = (int)(x / M % M), e = (int)(x / M / M % M), f = (int)(x / M / M / M);
+        ok = (z.re == 0 && z.im == 0) || a + c == e + f;
+    }
+    if (!ok) {
+        set_error("no conserved charge found: the couplings do not conserve the orbital-index sum");
+        return MCPI_ERR_STATE;
+    }
+    for (int j = 0; j < M; j++) charge[j] = j;
+    return MCPI_OK;
 }
 
 mcpi_status mcpi_hip_applyH_timed(mcpi_ham h, const mcpi_cplx *C_dev, mcpi_cplx *out_dev, int reps,

@@ -63,6 +63,9 @@ struct LzInst {
     double *partial_o;    // 2 * lm * nbg dot partials, t-major
     const double *z;      // lm Ritz coefficients (after the host solve)
     cplx *Cout;           // Ritz vector (may be null)
+    const int32_t *iperm; // sector-ordered basis: reference index of each row
+                          // (the start vector is gathered through it and the
+                          // Ritz vector scattered); null otherwise
     int nb, nbg;          // vector and dot blocks used by this instance
     int steps;            // Ritz length (after the host solve)
 };
@@ -165,7 +168,7 @@ __global__ __launch_bounds__(LZ_T) void lz_start(const LzInst *in)
     if ((int)blockIdx.x >= I.nb) return;
     LZ_ROWS(I, j)
     {
-        cplx x = ldv(gp(I.C0) + j);
+        cplx x = ldv(gp(I.C0) + (I.iperm ? (int64_t)I.iperm[j] : j));
         stv(gp(I.lvec) + j, x);
         stv(gp(I.vin) + j, x);
     }
@@ -469,7 +472,7 @@ __global__ __launch_bounds__(LZ_T) void lz_ritz(const LzInst *in)
             re += x.re * z[k];
             im += x.im * z[k];
         }
-        stv(gp(I.Cout) + j, cmake(re, im));
+        stv(gp(I.Cout) + (I.iperm ? (int64_t)I.iperm[j] : j), cmake(re, im));
     }
 }
 
@@ -503,6 +506,7 @@ static int sturm_count(int n, const double *d, const double *e2, double x, doubl
 // few ulps so that the two counts hold at the start.
 static double lowest_eigenvalue(int n, const double *d, const double *e)
 {
+    if (n == 1) return d[0];
     std::vector<double> e2(n > 1 ? n - 1 : 1, 0.0);
     double big = 1.0;
     for (int i = 0; i + 1 < n; i++) {
@@ -714,6 +718,14 @@ static mcpi_status tune_group(HvGroup &g, const mcpi_ham *hams, const std::vecto
 {
     std::vector<std::pair<int, int>> cand;   // (mode, sched)
     const bool full = g.basis->maps == MCPI_MAPS_FULL;
+    if (g.basis->sectored()) {
+        // rows in sector order: only the pattern tables hold their targets
+        if (!basis_pattern(g.basis, hams[g.members[0]]->c, true, s)) {
+            set_error("sector-ordered basis: the pattern tables could not be built (device memory short)");
+            return MCPI_ERR_STATE;
+        }
+        fixed = 4;
+    }
     for (int mode = 1; mode <= 2; mode++) {
         if (mode == 2 && !g.multi_ok) continue;
         if (g_batched_kernel == 1 && mode == 2) continue;
@@ -800,6 +812,8 @@ static mcpi_status lanczos_core(int K, const mcpi_ham *hams, const cplx *const *
             dense = dense && h->c.dense;
             same_zeros = same_zeros && h->c.zsig == h0->c.zsig && !h->c.dense;
         }
+        // a sector-ordered basis reads its pattern tables, one zero pattern per launch
+        if (g.basis->sectored() && !same_zeros) ok = false;
         g.batched = ok;
         if (ok) {
             g.skip = h0->skip_zero && !dense;
@@ -874,6 +888,7 @@ static mcpi_status lanczos_core(int K, const mcpi_ham *hams, const cplx *const *
         double *z = A.take<double>(lm);
         I.z = z;
         I.Cout = (Cout && Cout[x]) ? Cout[x] : nullptr;
+        I.iperm = hams[x]->basis->iperm;
         I.steps = 0;
     }
     for (HvGroup &g : groups) {
@@ -886,6 +901,15 @@ static mcpi_status lanczos_core(int K, const mcpi_ham *hams, const cplx *const *
     if (A.overflow) {
         set_error("internal error: Lanczos workspace layout");
         return MCPI_ERR_STATE;
+    }
+    // a start vector gathered in place would read rows already overwritten:
+    // gather from a copy in the (still unused) H.v output instead
+    for (int x = 0; x < K; x++) {
+        LzInst &I = hin[x];
+        if (I.iperm && (const void *)I.C0 == (const void *)I.lvec) {
+            MCPI_HIP(hipMemcpyAsync(I.HC, I.C0, sizeof(cplx) * I.n, hipMemcpyDeviceToDevice, s));
+            I.C0 = I.HC;
+        }
     }
     MCPI_HIP(hipMemcpyAsync(din, hin.data(), sizeof(LzInst) * K, hipMemcpyHostToDevice, s));
 
@@ -970,6 +994,17 @@ static mcpi_status lanczos_core(int K, const mcpi_ham *hams, const cplx *const *
     }
     if (diag_h) std::memcpy(diag_h, dg.data(), sizeof(cplx) * K * lm);
     if (offd_h) std::memcpy(offd_h, od.data(), sizeof(cplx) * K * lm);
+    // a caller's Krylov basis of a sector-ordered basis goes back to the
+    // reference order, row by row through the H.v output
+    for (int x = 0; x < K; x++) {
+        const LzInst &I = hin[x];
+        if (!(I.iperm && lvec_ext && lvec_ext[x])) continue;
+        for (int t = 0; t < steps[x]; t++) {
+            MCPI_CHECK(scatter_rows(I.n, I.iperm, I.lvec + (size_t)t * I.n, I.HC, s));
+            MCPI_HIP(hipMemcpyAsync(I.lvec + (size_t)t * I.n, I.HC, sizeof(cplx) * I.n, hipMemcpyDeviceToDevice, s));
+        }
+        MCPI_HIP(hipStreamSynchronize(s));
+    }
     if (!want_e) return MCPI_OK;
 
     // ---- tridiagonal eigenproblems on the host, Ritz vectors on the device

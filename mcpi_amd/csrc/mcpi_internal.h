@@ -105,6 +105,10 @@ struct BasisDev {
     // targets of the listed (non-zero) terms of each jump, in list order
     int32_t *patOT, *patTT;
     int64_t *ppoffOT, *ppoffTT;
+    // Sector order (null: the reference's order).  perm[i] = row of the
+    // state with reference index i; IF, Map and the pattern tables are in
+    // row order and hold row indices (builders apply perm to every rank).
+    const int32_t *perm;
 };
 
 // pattern tables of a basis for one zero pattern (see basis.hip)
@@ -162,6 +166,19 @@ struct mcpi_basis_s {
     void *rho_ws = nullptr;
     size_t rho_ws_bytes = 0;
     std::mutex rho_mu;
+    // Sector order (bases created with a conserved charge): rows sorted by
+    // the charge sum_j q_j v_j, reference order kept inside a sector.
+    // d.perm (device) maps reference index -> row, iperm row -> reference
+    // index; IF_ref is IF in the reference order; sec_off[s] .. sec_off[s+1]
+    // are the rows of the s-th sector (charges ascending); vt: two vectors
+    // of nc entries for moving vectors between the orders (allocated on
+    // first use).
+    int32_t *iperm = nullptr;
+    int32_t *IF_ref = nullptr;
+    std::vector<int32_t> charge;
+    std::vector<int64_t> sec_off;
+    cplx *vt[2] = {nullptr, nullptr};
+    bool sectored() const { return d.perm != nullptr; }
 };
 
 // Folded coefficient tables of a Hamiltonian (device memory), produced on
@@ -268,6 +285,12 @@ mcpi_status basis_upload_raw(int N, int M, const int32_t *IF, const int32_t *NCm
                              const int32_t *Map, const int32_t *MapOT, const int32_t *MapTT,
                              const int *strideOT, const int *strideTT, int64_t ot_len,
                              int64_t tt_len, mcpi_basis *out);
+// out[r] = in[idx[r]] for r < n (gather through an index table), on stream st
+mcpi_status gather_rows(int64_t n, const int32_t *idx, const cplx *in, cplx *out, hipStream_t st);
+// out[idx[r]] = in[r] for r < n (scatter through an index table)
+mcpi_status scatter_rows(int64_t n, const int32_t *idx, const cplx *in, cplx *out, hipStream_t st);
+// the basis's two order-conversion vectors (allocated on first use)
+mcpi_status basis_vectors(mcpi_basis b, cplx **v0, cplx **v1);
 // rank table B and NCmat of (N, M) in device memory, for kernels that run on
 // caller-owned tables (free both with hipFree)
 mcpi_status rank_tables_dev(int N, int M, int32_t **B, int32_t **NCmat);

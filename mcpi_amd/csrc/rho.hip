@@ -680,11 +680,25 @@ static mcpi_status density_scratch(mcpi_basis bs, int nblocks, int nterms, cplx 
     return MCPI_OK;
 }
 
+// The tables the density kernels read: the basis's own, or for a
+// sector-ordered basis the reference-order IF with no map (targets ranked
+// on the fly), so that densities take vectors in the reference order.
+static BasisDev density_view(mcpi_basis bs)
+{
+    BasisDev b = bs->d;
+    if (bs->sectored()) {
+        b.IF = bs->IF_ref;
+        b.Map = nullptr;
+        b.perm = nullptr;
+    }
+    return b;
+}
+
 // run one class and bring its compact terms to the host
 static mcpi_status run_class(mcpi_basis bs, int cls, const cplx *C, int p0, int p1, int nterms,
                              std::vector<cplx> &host)
 {
-    const BasisDev &b = bs->d;
+    const BasisDev b = density_view(bs);
     ClassPlan pl;
     MCPI_CHECK(plan_class(b, cls, nterms, &pl));
     cplx *partial = nullptr, *out = nullptr;
@@ -842,7 +856,7 @@ __global__ void k_expand_rho(int M, const cplx *__restrict__ A, const cplx *__re
 
 static mcpi_status vjp_noalloc(mcpi_basis bs, const cplx *C, cplx *g1, cplx *g2, hipStream_t s)
 {
-    const BasisDev &b = bs->d;
+    const BasisDev b = density_view(bs);
     const int M = b.M, P = M * (M - 1) / 2;
     const RepLayout L{M, P};
     const size_t cap = MCPI_NOALLOC_SCRATCH_BYTES / sizeof(cplx);

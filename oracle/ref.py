@@ -158,3 +158,20 @@ class RefBasis:
                        _ip_of(self.strideOT), _ip_of(self.strideTT), _ip_of(self.NCmat),
                        _ip_of(self.IF), _vp(C.ctypes.data), _vp(out.ctypes.data))
         return unflat(out, M, 4)
+
+
+def acth_omp_tables(N: int, M: int, Map, MapOT, MapTT, strideOT, strideTT, IF, C, Ho, Hint) -> np.ndarray:
+    """The reference's applyHconf_omp (hamiltonianMatrix.h:1431) on tables the
+    caller provides (int32, reference layout; strides int32 with nc entries),
+    e.g. tables copied from a device basis at sizes where the reference's
+    serial builders take minutes.  Uses the reference's OpenMP threads."""
+    L = lib()
+    i32 = lambda a: np.ascontiguousarray(a, np.int32)  # noqa: E731
+    Map, MapOT, MapTT, strideOT, strideTT, IF = map(i32, (Map, MapOT, MapTT, strideOT, strideTT, IF))
+    C = np.ascontiguousarray(C, np.complex128)
+    Hf = flat(Hint)
+    Hm = _Cmatrix(Ho)
+    out = np.zeros(C.size, np.complex128)
+    L.applyHconf_omp(N, M, _ip_of(Map), _ip_of(MapOT), _ip_of(MapTT), _ip_of(strideOT), _ip_of(strideTT),
+                     _ip_of(IF), _vp(C.ctypes.data), Hm.ptr, _vp(Hf.ctypes.data), _vp(out.ctypes.data))
+    return out

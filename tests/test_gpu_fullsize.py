@@ -96,12 +96,12 @@ def test_ground_n16_m12_lieb_liniger():
 
 def test_start_vector_from_null_stream_h_v(oracle_lib):
     """mcpi_hip_applyH(..., NULL) writes the start vector (here H = number
-    operator / N, so H C = C exactly) and mcpi_hip_ground reads it at once
-    from its own stream: the solver waits for the NULL stream first.  The
-    result equals a solve from a host-uploaded copy of the same vector, and
-    the oracle's."""
-    N, M, lm = 10, 12, 30
-    note("start vector written on the NULL stream, N=10 M=12")
+    operator / N with N = 8, so H C = C exactly: every v_k / 8 and their sum
+    are exact in binary) and mcpi_hip_ground reads it at once from its own
+    stream: the solver waits for the NULL stream first.  The result equals a
+    solve from a host-uploaded copy of the same vector, and the oracle's."""
+    N, M, lm = 8, 13, 30
+    note("start vector written on the NULL stream, N=8 M=13")
     b = mcpi_amd.Basis(N, M, "full")
     C0 = _start(b.nc, 77)
     Id = mcpi_amd.Hamiltonian(b, np.eye(M) / N, np.zeros((M,) * 4), schedule=2)
@@ -109,6 +109,9 @@ def test_start_vector_from_null_stream_h_v(oracle_lib):
     Hl = mcpi_amd.Hamiltonian(b, Ho, Hint)
     dC = mcpi_amd.DeviceArray.from_numpy(C0)
     E_ref, psi_ref, _ = Hl.ground(lm, dC)
+    dY = mcpi_amd.DeviceArray(b.nc)
+    Id.apply(dC, dY)
+    assert np.array_equal(dY.numpy(), C0)
     for _ in range(3):
         dX = mcpi_amd.DeviceArray(b.nc)
         Id.apply(dC, dX)                 # NULL stream, not synchronized
