@@ -217,6 +217,45 @@ def reference_build_baseline(target_s: float = 20.0):
             "rel_diff_reference_vs_gpu": rel, "rel_diff_port_vs_gpu": rel_port}
 
 
+def reference_shaped_calls(reps: int = 20):
+    """Per-call cost of the drop-in entry point applyHconf_omp
+    (hamiltonianMatrix.h:1431 shape: host tables, host vectors) at N=10,
+    M=12 (nc 352,716, Lieb-Liniger g=10), as a reference program calling it
+    in a loop pays it: the first call uploads the tables, folds the rules and
+    tunes the H.v schedule; later calls with the same tables and Ho/Hint
+    reuse all of that and pay the vector copies plus one H.v.  Beside it the
+    H.v alone on device vectors (mcpi_hip_applyH)."""
+    import mcpi_amd
+    import mcpi_amd.reference as R
+    N, M = 10, 12
+    t = time.perf_counter()
+    r = R.RefBasis(N, M)
+    build_s = time.perf_counter() - t
+    rng = np.random.default_rng(1012)
+    C = rng.standard_normal(r.nc) + 1j * rng.standard_normal(r.nc)
+    C /= np.linalg.norm(C)
+    Ho, Hint = mcpi_amd.ll_ho(M), mcpi_amd.ll_hint(M, G_)
+    t = time.perf_counter()
+    first = r.applyHconf_omp(C, Ho, Hint)
+    first_ms = (time.perf_counter() - t) * 1e3
+    t = time.perf_counter()
+    for _ in range(reps):
+        out = r.applyHconf_omp(C, Ho, Hint)
+    call_ms = (time.perf_counter() - t) / reps * 1e3
+    b = mcpi_amd.Basis(N, M, "full")
+    H = mcpi_amd.Hamiltonian(b, Ho, Hint)
+    dC, dO = mcpi_amd.DeviceArray.from_numpy(C), mcpi_amd.DeviceArray(b.nc)
+    H.apply(dC, dO)
+    dev_ms = H.apply_timed(dC, dO, reps)
+    same = float(np.abs(out - dO.numpy()).max() / np.abs(out).max())
+    R.release_cached()
+    return {"workload": "applyHconf_omp, N=10, M=12, Lieb-Liniger g=10, host tables from the "
+                        "reference-shaped builders", "nc": r.nc, "table_build_s": round(build_s, 2),
+            "first_call_ms": round(first_ms, 2), "repeated_call_ms": round(call_ms, 3),
+            "device_hv_ms": round(dev_ms, 4), "host_vector_copy_share": round(1 - dev_ms / call_ms, 3),
+            "max_rel_diff_vs_device_api": same, "first_vs_repeated_max_diff": float(np.abs(first - out).max())}
+
+
 def batched_lanczos(n_inst: int = 64, lm: int = 120, seed: int = 64120, serial: bool = True,
                     cpu_sample: bool = True):
     """64 Lieb-Liniger instances, (N, M) drawn from the seed over (6,9),
@@ -288,7 +327,18 @@ def batched_lanczos(n_inst: int = 64, lm: int = 120, seed: int = 64120, serial: 
             t_gpu = time.perf_counter() - t
             cpu["per_basis"][f"({N},{M})"] = {"cpu_s_per_gs": round(t_cpu, 4), "gpu_s_per_gs": round(t_gpu, 5),
                                             "abs_dE0_cpu_vs_gpu": float(abs(E_cpu - E_gpu))}
+    # Bytes the two classical Gram-Schmidt passes must move in any solver
+    # that reorthogonalizes like the reference (groundStateLieb.c:316-332):
+    # step i reads the i+2 basis vectors once for the dots and once for the
+    # update.  Batched and per-instance solves both pay them, so they bound
+    # the batched speed-up from above.
+    ncs = np.array([hams[x].basis.nc for x in range(n_inst)], dtype=np.float64)
+    gs_bytes = float(32.0 * ncs.sum() * (lm * (lm + 1) / 2 - 1))
+    gs_floor_s = gs_bytes / (HBM_PEAK_GBS * 1e9)
     return {
+        "gram_schmidt_bytes": gs_bytes,
+        "gram_schmidt_floor_s_at_hbm_peak": round(gs_floor_s, 4),
+        "speedup_ceiling_at_hbm_peak": (round(t_serial / gs_floor_s, 2) if serial else None),
         "workload": (f"{n_inst} Lieb-Liniger instances, (N,M) from the seed over (6,9)/(8,11)/(10,13) "
                      f"{counts}, g log-uniform [0.1,100], lm={lm}, full maps, complex128"),
         "gs_per_s_batched": round(n_inst / t_batched, 3),
@@ -506,6 +556,9 @@ def main() -> int:
             mcpi_amd.synchronize()
             dens[label] = round((time.perf_counter() - t) / 5 * 1e3, 3)
 
+    refcalls = None
+    if not args.profile:
+        refcalls = reference_shaped_calls()
     batched = large = None
     if not (args.no_batched or args.profile):
         del dO
@@ -557,6 +610,7 @@ def main() -> int:
             "basis_build_s": round(build_s, 3),
             "other_schedules": others,
             "densities_n12_m12": dens,
+            "reference_shaped_applyHconf_omp_n10_m12": refcalls,
             "batched_lanczos": batched,
             "large_basis_n16_m12": large,
             "max_rel_diff_between_schedules": agree,
