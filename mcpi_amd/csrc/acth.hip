@@ -277,10 +277,12 @@ __global__ __launch_bounds__(256) void k_acth_rows(BasisDev b, HamCoefDev c, con
     // single jumps k -> l: 1-body off-diagonal, rules 4, 5, 8
     for (int k = 0; k < M; k++) {
         const int vk = V(k);
-        for (int l = 0; l < M; l++) {
-            if (l == k) continue;
+        // l with a non-zero coefficient (every l != k without zero-skipping)
+        uint32_t lmask = skip_zero ? c.m1j[k] : (~0u >> (32 - M)) & ~(1u << k);
+        while (lmask) {
+            const int l = __ffs(lmask) - 1;
+            lmask &= lmask - 1;
             const int kl = k + M * l;
-            if (skip_zero && !c.nz_1j[kl]) continue;
             if (vk < 1) continue;
             cplx h = c.A0[kl];
             const cplx *Ak = c.A + (int64_t)kl * M;
@@ -297,7 +299,9 @@ __global__ __launch_bounds__(256) void k_acth_rows(BasisDev b, HamCoefDev c, con
         }
     }
 
-    // same-orbital double jumps: rules 3 and 6
+    // same-orbital double jumps: rules 3 and 6.  With zero-skipping the
+    // chunk's list of non-zero entries is walked (HamCoefDev::ot_off / nz_e)
+    // instead of testing all M(M+1)/2 entries.
     int64_t cot = 0;
     for (int k = 0; k < M; k++) {
         const int vk = V(k);
@@ -305,12 +309,22 @@ __global__ __launch_bounds__(256) void k_acth_rows(BasisDev b, HamCoefDev c, con
         const double fk = sq[vk] * sq[vk - 1];
         const int64_t base = (MODE == 2) ? b.strideOT[i] + cot * MM : 0;
         cot++;
+        if (skip_zero) {
+            for (int x0 = c.ot_off[k], x1 = c.ot_off[k + 1]; x0 < x1; x0++) {
+                const int e = c.nz_e[x0], q = e % M, l = e / M;
+                double x = (q == l) ? sq[V(q) + 1] * sq[V(q) + 2] : sq[V(q) + 1] * sq[V(l) + 1];
+                int64_t j;
+                if (MODE == 2) j = b.MapOT[base + e];
+                else j = rank_lds(Ps, T, t, Bt, N, M, nc, k, k, q, l);
+                acc = cfma(c.nz_c[x0], cscale(ldc(C + j), fk * x), acc);
+            }
+            continue;
+        }
         for (int l = 0; l < M; l++) {
             if (l == k) continue;
             for (int q = 0; q <= l; q++) {
                 if (q == k) continue;
                 const int e = q + M * l;
-                if (skip_zero && !c.nz_ot[k * MM + e]) continue;
                 double x = (q == l) ? sq[V(q) + 1] * sq[V(q) + 2] : sq[V(q) + 1] * sq[V(l) + 1];
                 int64_t j;
                 if (MODE == 2) j = b.MapOT[base + e];
@@ -331,14 +345,23 @@ __global__ __launch_bounds__(256) void k_acth_rows(BasisDev b, HamCoefDev c, con
             const double fks = sq[vk] * sq[vs];
             const int64_t base = (MODE == 2) ? b.strideTT[i] + ctt * MM : 0;
             ctt++;
+            if (skip_zero) {
+                for (int x0 = c.tt_off[p], x1 = c.tt_off[p + 1]; x0 < x1; x0++) {
+                    const int e = c.nz_e[x0], q = e % M, l = e / M;
+                    double x = (q == l) ? sq[V(q) + 1] * sq[V(q) + 2] : sq[V(q) + 1] * sq[V(l) + 1];
+                    int64_t j;
+                    if (MODE == 2) j = b.MapTT[base + e];
+                    else j = rank_lds(Ps, T, t, Bt, N, M, nc, k, s, q, l);
+                    acc = cfma(c.nz_c[x0], cscale(ldc(C + j), fks * x), acc);
+                }
+                continue;
+            }
             const cplx *cp = c.tt + (int64_t)p * MM;
-            const uint8_t *np = c.nz_tt + (int64_t)p * MM;
             for (int l = 0; l < M; l++) {
                 if (l == k || l == s) continue;
                 for (int q = 0; q <= l; q++) {
                     if (q == k || q == s) continue;
                     const int e = q + M * l;
-                    if (skip_zero && !np[e]) continue;
                     double x = (q == l) ? sq[V(q) + 1] * sq[V(q) + 2] : sq[V(q) + 1] * sq[V(l) + 1];
                     int64_t j;
                     if (MODE == 2) j = b.MapTT[base + e];

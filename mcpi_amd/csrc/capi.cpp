@@ -720,12 +720,17 @@ void IndexToFock(int k, int N, int M, Iarray v)
 
 int FockToIndex(int N, int M, Iarray NCmat, Iarray v)
 {
-    int k = 0, col = N + 1;
-    for (int i = M - 1; i > 0; i--)
-        for (int n = v[i]; n > 0; n--) {
-            k += NCmat[N + col * i];
-            N--;
-        }
+    // Closed form of the reference's loop: the particles of orbital i add
+    // NC(n, i) for n from P_i + 1 to P_{i+1} (P_i = particles below orbital
+    // i), and by the hockey-stick identity that sum is
+    // NC(P_{i+1}, i+1) - NC(P_i, i+1), read from the caller's table.
+    const int col = N + 1;
+    int k = 0, below = v[0];
+    for (int i = 1; i < M; i++) {
+        const int upto = below + v[i];
+        k += NCmat[upto + col * (i + 1)] - NCmat[below + col * (i + 1)];
+        below = upto;
+    }
     return k;
 }
 

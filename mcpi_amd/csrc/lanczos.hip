@@ -63,6 +63,9 @@ struct LzInst {
     double *partial_o;    // 2 * lm * nbg dot partials, t-major
     const double *z;      // lm Ritz coefficients (after the host solve)
     cplx *Cout;           // Ritz vector (may be null)
+    cplx *rpart;          // small bases: per-vector-group partial updates of
+                          // the Gram-Schmidt pass (ceil(lm / LZ_TG) x n), null
+                          // for the single-pass update
     const int32_t *iperm; // sector-ordered basis: reference index of each row
                           // (the start vector is gathered through it and the
                           // Ritz vector scattered); null otherwise
@@ -408,7 +411,7 @@ __global__ __launch_bounds__(LZ_T) void lz_reorth_norm(const LanczosState *sts, 
 {
     const LanczosState *st = sts + blockIdx.y;
     const LzInst &I = in[blockIdx.y];
-    if ((int)blockIdx.x >= I.nb || st->broke) return;
+    if ((int)blockIdx.x >= I.nb || st->broke || I.rpart) return;
     const int nt = st->it + 2;
     const int64_t n = I.n;
     const LZ_CONST cplx *o = kp(I.ortho);
@@ -445,6 +448,70 @@ __global__ __launch_bounds__(LZ_T) void lz_reorth_norm(const LanczosState *sts, 
                 s += h[r].re * h[r].re + h[r].im * h[r].im;
             }
         }
+    }
+    block_sum2(s, 0, I.partial);
+}
+
+// 8 on small bases.  With few rows, a pass in which every thread walks all
+// i+2 basis vectors for its rows has too few threads to keep the memory
+// busy (N=5, M=21: 52 blocks for up to 6,641 vectors).  The update is split
+// like the dots: block (s, g, instance) sums v_t o_t over the vector group g
+// for the rows of segment s into rpart[g] (8a), then one pass subtracts the
+// groups' partial sums from HC and forms the norm partials (8b).
+__global__ __launch_bounds__(LZ_T) void lz_reorth_part(const LanczosState *sts, const LzInst *in)
+{
+    const LanczosState *st = sts + blockIdx.z;
+    const LzInst &I = in[blockIdx.z];
+    if (!I.rpart || (int)blockIdx.x >= I.nbg || st->broke) return;
+    const int nt = st->it + 2;
+    const int t0 = blockIdx.y * LZ_TG;
+    if (t0 >= nt) return;
+    const int t1 = min(nt, t0 + LZ_TG);
+    const int64_t n = I.n;
+    const int64_t base = (int64_t)blockIdx.x * LZ_SEG + threadIdx.x;
+    const LZ_CONST cplx *o = kp(I.ortho);
+    cplx p[LZ_R];
+#pragma unroll
+    for (int r = 0; r < LZ_R; r++) p[r] = cmake(0, 0);
+    for (int tt = t0; tt < t1; tt++) {
+        const cplx ot = cmake(o[tt].re, o[tt].im);
+        const LZ_GLOBAL cplx *vt = gp(I.lvec) + (int64_t)tt * n;
+#pragma unroll
+        for (int r = 0; r < LZ_R; r++) {
+            const int64_t j = base + (int64_t)r * LZ_T;
+            if (j < n) {
+                const cplx x = ldv_nt(vt + j);
+                p[r].re += x.re * ot.re - x.im * ot.im;
+                p[r].im += x.re * ot.im + x.im * ot.re;
+            }
+        }
+    }
+    LZ_GLOBAL cplx *dst = gp(I.rpart) + (int64_t)blockIdx.y * n;
+#pragma unroll
+    for (int r = 0; r < LZ_R; r++) {
+        const int64_t j = base + (int64_t)r * LZ_T;
+        if (j < n) stv(dst + j, p[r]);
+    }
+}
+
+__global__ __launch_bounds__(LZ_T) void lz_reorth_sum(const LanczosState *sts, const LzInst *in)
+{
+    const LanczosState *st = sts + blockIdx.y;
+    const LzInst &I = in[blockIdx.y];
+    if (!I.rpart || (int)blockIdx.x >= I.nb || st->broke) return;
+    const int ng = (st->it + 2 + LZ_TG - 1) / LZ_TG;
+    const int64_t n = I.n;
+    double s = 0;
+    LZ_ROWS(I, j)
+    {
+        cplx h = ldv(gp(I.HC) + j);
+        for (int g = 0; g < ng; g++) {
+            const cplx x = ldv(gp(I.rpart) + (int64_t)g * n + j);
+            h.re -= x.re;
+            h.im -= x.im;
+        }
+        stv(gp(I.HC) + j, h);
+        s += h.re * h.re + h.im * h.im;
     }
     block_sum2(s, 0, I.partial);
 }
@@ -607,6 +674,11 @@ mcpi_status tridiag_lowest(int n, const double *d, const double *e, double *lamb
 }
 
 // ---------------------------------------------------------------- host --
+
+// the Gram-Schmidt update of an instance with fewer than 256 row blocks is
+// split over vector groups (lz_reorth_part / lz_reorth_sum) when there are
+// more vectors than one group
+static bool split_update(int nb, int lm) { return nb < 256 && lm > LZ_TG; }
 
 static int vec_blocks(int64_t n)
 {
@@ -845,6 +917,7 @@ static mcpi_status lanczos_core(int K, const mcpi_ham *hams, const cplx *const *
         A.reserve(sizeof(double) * 2 * I.nb);
         A.reserve(sizeof(double) * 2 * (size_t)I.nbg * lm);
         A.reserve(sizeof(double) * lm);
+        if (split_update(I.nb, lm)) A.reserve(sizeof(cplx) * n * (size_t)((lm + LZ_TG - 1) / LZ_TG));
     }
     for (HvGroup &g : groups) {
         if (!g.batched) continue;
@@ -853,6 +926,7 @@ static mcpi_status lanczos_core(int K, const mcpi_ham *hams, const cplx *const *
         for (int k = 0; k < 2; k++) A.reserve(sizeof(cplx) * (size_t)g.basis->d.nc * m);
     }
     MCPI_CHECK(A.allocate());
+    bool any_split = false;
     LanczosState *sts = A.take<LanczosState>(K);
     LzInst *din = A.take<LzInst>(K);
     for (HvGroup &g : groups) {
@@ -887,6 +961,8 @@ static mcpi_status lanczos_core(int K, const mcpi_ham *hams, const cplx *const *
         I.partial_o = A.take<double>(2 * (size_t)I.nbg * lm);
         double *z = A.take<double>(lm);
         I.z = z;
+        I.rpart = split_update(I.nb, lm) ? A.take<cplx>(n * (size_t)((lm + LZ_TG - 1) / LZ_TG)) : nullptr;
+        any_split = any_split || I.rpart != nullptr;
         I.Cout = (Cout && Cout[x]) ? Cout[x] : nullptr;
         I.iperm = hams[x]->basis->iperm;
         I.steps = 0;
@@ -946,6 +1022,10 @@ static mcpi_status lanczos_core(int K, const mcpi_ham *hams, const cplx *const *
             hipLaunchKernelGGL(lz_gemv_dots, gd, blk, 0, s, cst, cin, lm);
             hipLaunchKernelGGL(lz_fin_ortho, go, blk, 0, s, cst, cin);
             hipLaunchKernelGGL(lz_reorth_norm, gv, blk, 0, s, cst, cin);
+            if (any_split) {
+                hipLaunchKernelGGL(lz_reorth_part, gd, blk, 0, s, cst, cin);
+                hipLaunchKernelGGL(lz_reorth_sum, gv, blk, 0, s, cst, cin);
+            }
             hipLaunchKernelGGL(lz_advance, dim3(ga), dim3(64), 0, s, sts, K);
             MCPI_HIP(hipGetLastError());
             return MCPI_OK;
