@@ -47,6 +47,21 @@ def algorithmic_bytes(nc: int, M: int, ot_len: int, tt_len: int) -> int:
 KERNEL_SOURCES = ["mcpi_amd/csrc/acth_comp.hip", "mcpi_amd/csrc/acth_fast.h", "mcpi_amd/csrc/mcpi_internal.h"]
 
 
+def density_model(N: int, M: int, rho2_ms: float) -> dict:
+    """Work of the Gram form of rho2: Phi (rows: the states of N-2
+    particles, columns: the M(M+1)/2 orbital pairs) gathered from C, and the
+    upper triangle of Phi^H Phi at 8 real flops per complex multiply-add;
+    rate per host call against the FP64 peak."""
+    from math import comb
+    nu = comb(N - 2 + M - 1, M - 1) if N >= 2 else 0
+    npair = M * (M + 1) // 2
+    flops = 8 * nu * npair * (npair + 1) // 2
+    return {"rho2_rows": nu, "rho2_pairs": npair, "rho2_gathers": nu * npair, "rho2_flops": flops,
+            "rho2_tflops_per_call": round(flops / (rho2_ms * 1e-3) / 1e12, 2),
+            "rho2_fp64_peak_tflops": 78.6,
+            "rho2_frac_fp64_per_call": round(flops / (rho2_ms * 1e-3) / 78.6e12, 3)}
+
+
 def kernel_sources_sha() -> str:
     import hashlib
     h = hashlib.sha256()
@@ -542,8 +557,8 @@ def main() -> int:
             agree = max(agree, float(np.linalg.norm(dO.numpy() - gpu_out) / np.linalg.norm(gpu_out)))
 
     # one- and two-body densities of the same vector on the same basis
-    # (calls include the device-to-host copy of the representative terms
-    # and the host's symmetry expansion)
+    # (per call from the host: the Gram kernels, the block sums, the
+    # expansion into the full matrices and their copy to the host)
     dens = gpu_rho = None
     if not args.profile:
         gpu_rho = (b.rho1(dC), b.rho2(dC))
@@ -551,10 +566,11 @@ def main() -> int:
         dens = {}
         for label, fn in (("rho1_ms", b.rho1), ("rho2_ms", b.rho2)):
             t = time.perf_counter()
-            for _ in range(5):
+            for _ in range(20):
                 fn(dC)
             mcpi_amd.synchronize()
-            dens[label] = round((time.perf_counter() - t) / 5 * 1e3, 3)
+            dens[label] = round((time.perf_counter() - t) / 20 * 1e3, 4)
+        dens.update(density_model(N_, M_, dens["rho2_ms"]))
 
     refcalls = None
     if not args.profile:

@@ -335,9 +335,11 @@ mcpi_status mcpi_hip_vjp(mcpi_basis b, const mcpi_cplx *C_dev, mcpi_cplx *dE_dHo
  * of mcpi_hip_vjp with device outputs (dE_dHo_dev M*M, dE_dHint_dev M^4),
  * enqueued on stream (NULL = default) without host synchronization.  Its
  * partial sums live in a 16 MiB device array of the library (one per GPU,
- * held until the process ends); it calls no hipMalloc/hipFree.  Bases beyond
- * the density kernels' limit (M > 24) give MCPI_ERR_ARG.  Calls that
- * overlap on one GPU share that array and race: serialize them. */
+ * held until the process ends); it calls no hipMalloc/hipFree.  Every basis
+ * the library builds (M <= 32) fits: the Gram tiles take at most 2.9 MiB and
+ * the number of block partials is cut to what remains; a basis that would
+ * not fit gives MCPI_ERR_ARG.  Calls that overlap on one GPU share that
+ * array and race: serialize them. */
 mcpi_status mcpi_hip_vjp_noalloc(mcpi_basis b, const mcpi_cplx *C_dev, mcpi_cplx *dE_dHo_dev,
                                  mcpi_cplx *dE_dHint_dev, void *stream);
 

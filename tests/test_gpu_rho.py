@@ -12,17 +12,15 @@ from tests.gpu_util import assert_close, note
 pytestmark = pytest.mark.gpu
 
 
-@pytest.mark.parametrize("N,M", [(1, 3), (2, 3), (5, 5), (4, 6), (6, 4), (3, 7), (7, 9), (3, 12), (2, 14), (2, 17),
-                                 (2, 24)])
+@pytest.mark.parametrize("N,M", [(1, 3), (2, 3), (5, 5), (4, 6), (6, 4), (3, 7), (7, 9), (3, 12), (3, 13), (2, 14),
+                                 (2, 17), (2, 24), (2, 32)])
 @pytest.mark.parametrize("maps", ["full", "single", "none"])
 def test_rho_matches_oracle(oracle_lib, N, M, maps):
-    """Targets read from the stored maps (full), or ranked on the fly for the
-    double jumps (single) or for every jump (none).  M = 12 and 14: same-orbital
-    jumps with more than 64 terms (two passes of the lanes); M = 14, 17: the
-    two-orbital accumulators split over pair ranges; M = 17, 24: ranked
-    targets per term instead of per-row difference tables (M > 16); M = 24:
-    the largest M whose class-0 accumulators fit in LDS, double-jump blocks
-    narrowed to leave room for them."""
+    """Every storage mode (the densities read no map).  Tiles of the Gram
+    matrix of rho2 (80 pair columns a side): M = 12 fills one tile (78 pairs),
+    M = 13 (91) needs two and the off-diagonal tile pair, M = 24 (300) four
+    tiles a side, M = 32 (528) seven; N = 1: rho2 = 0, rho1 from the
+    one-particle Gram matrix."""
     note(f"rho N={N} M={M} maps={maps}")
     o = oracle.Basis(N, M)
     C, _, _ = oracle.random_inputs(N, M, seed=21 + N)
@@ -108,9 +106,10 @@ def test_vjp_finite_differences(oracle_lib):
 
 
 def test_rho_single_map_n16_m12_invariants(oracle_lib):
-    """N=16, M=12 with only Map stored (nc 13,037,895): rho2's double jumps
-    ranked on the fly; partial traces, contraction, Hermiticity, and the
-    energy identity against H.v of the same basis."""
+    """N=16, M=12 with only Map stored (nc 13,037,895; the Gram matrix of
+    rho2 runs over the 4,457,400 states of 14 particles): partial traces,
+    contraction, Hermiticity, and the energy identity against H.v of the
+    same basis."""
     N, M = 16, 12
     note("rho invariants N=16 M=12 single map")
     b = mcpi_amd.Basis(N, M, "single")
