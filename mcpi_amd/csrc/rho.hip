@@ -135,7 +135,7 @@ __host__ __device__ inline GramLds gram_lds(int N, int M, int np, int rs, bool d
 // Block partials of one tile pair of Phi^H Phi.  DIAG: tile pair (TI, TI)
 // from blockIdx.y, sub-blocks i <= j; else the off-diagonal pair
 // off_pair(blockIdx.y), all sub-blocks.
-template <bool DIAG, int GU>
+template <bool DIAG, int GU, int NG>
 __global__ __launch_bounds__(GTHREADS) __attribute__((amdgpu_waves_per_eu(DIAG ? 3 : 2))) void k_gram(GramArgs a)
 {
     extern __shared__ __align__(16) unsigned char lds[];
@@ -168,20 +168,19 @@ __global__ __launch_bounds__(GTHREADS) __attribute__((amdgpu_waves_per_eu(DIAG ?
             ql[x] = (uint16_t)(x | (x << 8));
         }
     }
-    // columns of the two sides: groups of 16 holding a column of Phi
+    // columns of the two sides (NG groups of 16, a.ng == NG; the columns of
+    // a narrower last tile past wA / wB hold zeros, so every sub-block is
+    // computed without a test)
     const int wA = min(GT, a.np - TI * GT), wB = min(GT, a.np - TJ * GT);
-    const int ngA = (wA + 15) / 16, ngB = (wB + 15) / 16;
     const int r = t >> 4, c = t & 15;
 
-    // column groups a tile side can hold: rho1 has at most 32 columns
-    constexpr int NG = GU == GU1 ? 2 : GG;
     cplx acc[NG][NG];
 #pragma unroll
     for (int i = 0; i < NG; i++)
 #pragma unroll
         for (int j = 0; j < NG; j++) acc[i][j] = cmake(0, 0);
 
-    const int ncolA = 16 * ngA, ncolB = DIAG ? 0 : 16 * ngB;
+    constexpr int ncolA = 16 * NG, ncolB = DIAG ? 0 : 16 * NG;
     const int nent = GU * (ncolA + ncolB);
     for (int64_t tile = blockIdx.x; tile < a.nut; tile += gridDim.x) {
         __syncthreads();   // the previous tile's Gram reads are done
@@ -191,13 +190,21 @@ __global__ __launch_bounds__(GTHREADS) __attribute__((amdgpu_waves_per_eu(DIAG ?
             const int64_t u = u0 + t;
             uint8_t *P = pp + t * (M + 1);
             if (u < a.nu) {
+                // one step per iteration (lower p, or settle orbital j and
+                // move down), so the lanes of a wavefront stay together:
+                // at most Np + M - 1 iterations
                 int64_t R = a.nu - 1 - u;
-                int pj = Np;
+                int pj = Np, j = M - 1;
                 P[M] = (uint8_t)Np;
-                for (int j = M - 1; j >= 1; j--) {
-                    while (Bt[j * NB + pj] > R) pj--;
-                    P[j] = (uint8_t)pj;
-                    R -= Bt[j * NB + pj];
+                while (j >= 1) {
+                    const int64_t bj = Bt[j * NB + pj];
+                    if (bj > R) {
+                        pj--;
+                    } else {
+                        P[j] = (uint8_t)pj;
+                        R -= bj;
+                        j--;
+                    }
                 }
                 P[0] = 0;
                 int64_t s0 = 0, s1 = 0, s2 = 0;
@@ -283,15 +290,15 @@ __global__ __launch_bounds__(GTHREADS) __attribute__((amdgpu_waves_per_eu(DIAG ?
             cplx av[NG], bv[NG];
 #pragma unroll
             for (int i = 0; i < NG; i++) {
-                if (i < ngA) av[i] = ra[16 * i + r];
-                if (i < (DIAG ? ngA : ngB)) bv[i] = rb[16 * i + c];
+                av[i] = ra[16 * i + r];
+                bv[i] = rb[16 * i + c];
             }
 #pragma unroll
             for (int i = 0; i < NG; i++)
 #pragma unroll
                 for (int j = 0; j < NG; j++) {
                     if (DIAG && j < i) continue;
-                    if (i < ngA && j < (DIAG ? ngA : ngB)) acc[i][j] = cfma_conj(av[i], bv[j], acc[i][j]);
+                    acc[i][j] = cfma_conj(av[i], bv[j], acc[i][j]);
                 }
         }
     }
@@ -414,31 +421,43 @@ static mcpi_status plan_gram(const BasisDev &b, int ord, int max_blocks, GramPla
     return MCPI_OK;
 }
 
+template <bool DIAG, int U, int NG>
+static mcpi_status launch_gram(const GramPlan &pl, int ny, size_t sm, hipStream_t s)
+{
+    (void)hipFuncSetAttribute((const void *)k_gram<DIAG, U, NG>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)sm);
+    hipLaunchKernelGGL((k_gram<DIAG, U, NG>), dim3(pl.nblocks, ny), dim3(GTHREADS), sm, s, pl.a);
+    MCPI_HIP(hipGetLastError());
+    return MCPI_OK;
+}
+
+// the block-partial kernels of a plan: rho1 (U = GU1, one tile of at most
+// 32 columns) or rho2 (U = GU2: the diagonal tile pairs, then the others)
+static mcpi_status launch_gram_kernels(const GramPlan &pl, hipStream_t s)
+{
+    const int T = pl.a.ntile;
+    if (pl.U == GU1) {
+        MCPI_ARG(T == 1 && pl.a.ng <= 2, "one-body Gram matrix wider than two column groups");
+        return pl.a.ng == 1 ? launch_gram<true, GU1, 1>(pl, 1, pl.sm_diag, s)
+                            : launch_gram<true, GU1, 2>(pl, 1, pl.sm_diag, s);
+    }
+    switch (pl.a.ng) {
+    case 1: MCPI_CHECK((launch_gram<true, GU2, 1>(pl, T, pl.sm_diag, s))); break;
+    case 2: MCPI_CHECK((launch_gram<true, GU2, 2>(pl, T, pl.sm_diag, s))); break;
+    case 3: MCPI_CHECK((launch_gram<true, GU2, 3>(pl, T, pl.sm_diag, s))); break;
+    case 4: MCPI_CHECK((launch_gram<true, GU2, 4>(pl, T, pl.sm_diag, s))); break;
+    default: MCPI_CHECK((launch_gram<true, GU2, GG>(pl, T, pl.sm_diag, s))); break;
+    }
+    // off-diagonal tile pairs exist only when the columns fill more than one tile
+    if (T > 1) MCPI_CHECK((launch_gram<false, GU2, GG>(pl, T * (T - 1) / 2, pl.sm_off, s)));
+    return MCPI_OK;
+}
+
 // Phi^H Phi into G (pl.gram_entries()) through partial, on stream s
 static mcpi_status run_gram(GramPlan pl, const cplx *C, cplx *partial, cplx *G, hipStream_t s)
 {
     pl.a.C = C;
     pl.a.partial = partial;
-    const int T = pl.a.ntile;
-    if (pl.U == GU1) {   // rho1: at most 32 columns, one tile
-        MCPI_ARG(T == 1, "one-body Gram matrix wider than one tile");
-        (void)hipFuncSetAttribute((const void *)k_gram<true, GU1>, hipFuncAttributeMaxDynamicSharedMemorySize,
-                                  (int)pl.sm_diag);
-        hipLaunchKernelGGL((k_gram<true, GU1>), dim3(pl.nblocks, 1), dim3(GTHREADS), pl.sm_diag, s, pl.a);
-        MCPI_HIP(hipGetLastError());
-    } else {
-        (void)hipFuncSetAttribute((const void *)k_gram<true, GU2>, hipFuncAttributeMaxDynamicSharedMemorySize,
-                                  (int)pl.sm_diag);
-        hipLaunchKernelGGL((k_gram<true, GU2>), dim3(pl.nblocks, T), dim3(GTHREADS), pl.sm_diag, s, pl.a);
-        MCPI_HIP(hipGetLastError());
-        if (T > 1) {
-            (void)hipFuncSetAttribute((const void *)k_gram<false, GU2>,
-                                      hipFuncAttributeMaxDynamicSharedMemorySize, (int)pl.sm_off);
-            hipLaunchKernelGGL((k_gram<false, GU2>), dim3(pl.nblocks, T * (T - 1) / 2), dim3(GTHREADS), pl.sm_off,
-                               s, pl.a);
-            MCPI_HIP(hipGetLastError());
-        }
-    }
+    MCPI_CHECK(launch_gram_kernels(pl, s));
     const int64_t n = (int64_t)pl.ntp * GTILE;
     const int seg = (pl.nblocks + GSEG - 1) / GSEG, nseg = (pl.nblocks + seg - 1) / seg;
     hipLaunchKernelGGL(k_gram_sum1, dim3((unsigned)((n + 255) / 256), (unsigned)nseg), dim3(256), 0, s, pl.ntp,
