@@ -30,6 +30,30 @@ def test_rho_matches_oracle(oracle_lib, N, M, maps):
     assert_close(b.rho2(dC), o.rho2(C), 1e-12, "rho2")
 
 
+@pytest.mark.parametrize("N,M", [(0, 3), (1, 1), (2, 1), (4, 1), (3, 2), (12, 2), (30, 3)])
+@pytest.mark.parametrize("maps", ["full", "none"])
+def test_rho_edge_shapes(oracle_lib, N, M, maps):
+    """No particle (one state, both densities zero), one orbital (rho1 = N,
+    rho2 = N(N-1): a single unranked row of N-1 or N-2 particles), two
+    orbitals, and many particles in few orbitals (N = 30, M = 3: the rank
+    table's widest rows)."""
+    note(f"rho edge N={N} M={M} maps={maps}")
+    o = oracle.Basis(N, M)
+    C, _, _ = oracle.random_inputs(N, M, seed=40 + N)
+    b = mcpi_amd.Basis(N, M, maps)
+    dC = mcpi_amd.DeviceArray.from_numpy(C)
+    r1, r2 = b.rho1(dC), b.rho2(dC)
+    ref1, ref2 = o.rho1(C), o.rho2(C)
+    if N == 0:
+        assert not r1.any() and not r2.any()
+        return
+    assert_close(r1, ref1, 1e-12, "rho1")
+    if N == 1:
+        assert not r2.any()
+    else:
+        assert_close(r2, ref2, 1e-12, "rho2")
+
+
 def test_rho_golden():
     gd = np.load(os.path.join(os.path.dirname(__file__), "golden", "cfg1_perf.npz"))
     note("rho golden cfg1")
