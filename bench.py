@@ -25,6 +25,7 @@ from __future__ import annotations
 
 import argparse
 import json
+import math
 import os
 import sys
 import time
@@ -60,6 +61,14 @@ def density_model(N: int, M: int, rho2_ms: float) -> dict:
             "rho2_tflops_per_call": round(flops / (rho2_ms * 1e-3) / 1e12, 2),
             "rho2_fp64_peak_tflops": 78.6,
             "rho2_frac_fp64_per_call": round(flops / (rho2_ms * 1e-3) / 78.6e12, 3)}
+
+
+def pair_product_flops(N: int, M: int) -> int:
+    """Real FP64 flops of the pair-space product Y2 = W X of one H.v:
+    6 NP^2 nu2 (three real products of 2 flops per complex entry), NP =
+    M(M+1)/2 orbital pairs, nu2 = C(N-2+M-1, N-2) states of N-2 particles."""
+    NP = M * (M + 1) // 2
+    return 6 * NP * NP * math.comb(N - 2 + M - 1, N - 2)
 
 
 def kernel_sources_sha() -> str:
@@ -389,13 +398,24 @@ def large_basis(reps: int = 20, lm: int = 200, quick: bool = False):
     Ho, Hint = mcpi_amd.ll_ho(M), mcpi_amd.ll_hint(M, g)
     out = {"workload": "N=16, M=12, Lieb-Liniger g=50, Map stored, double jumps ranked",
            "nc": nc, "bytes_per_hv": B, "basis_build_s": round(build_s, 3)}
-    for label, skip in ((("skip_zero", True),) if quick else (("every_term", False), ("skip_zero", True))):
-        H = mcpi_amd.Hamiltonian(b, Ho, Hint, skip_zero=skip)
+    # every term reading Map (schedule 2, the byte model's path), every term
+    # through the pair space (schedule 6: no map read, so no byte model), and
+    # the autotuned zero-skipping path
+    legs = (("skip_zero", dict(skip_zero=True)),)
+    if not quick:
+        legs = (("every_term", dict(skip_zero=False, schedule=2)),
+                ("every_term_pair_space", dict(skip_zero=False, schedule=6))) + legs
+    for label, kw in legs:
+        H = mcpi_amd.Hamiltonian(b, Ho, Hint, **kw)
         H.apply(dC, dO)                      # schedule tuning
         ms = H.apply_timed(dC, dO, reps)
-        out[label] = {"ms_per_hv": round(ms, 4), "gb_s_model": round(B / ms / 1e6, 1),
-                      "hbm_frac_model": round(B / ms / 1e6 / HBM_PEAK_GBS, 4), "schedule": H.schedule,
-                      "pattern_table_bytes": b.pattern_bytes()}
+        if H.schedule == 6:
+            out[label] = {"ms_per_hv": round(ms, 4), "schedule": 6,
+                          "pair_product_flops": pair_product_flops(N, M)}
+        else:
+            out[label] = {"ms_per_hv": round(ms, 4), "gb_s_model": round(B / ms / 1e6, 1),
+                          "hbm_frac_model": round(B / ms / 1e6 / HBM_PEAK_GBS, 4), "schedule": H.schedule,
+                          "pattern_table_bytes": b.pattern_bytes()}
         H.close()
     # the same H.v on the sector-ordered basis (rows sorted by total
     # momentum, which the Lieb-Liniger Hamiltonian conserves): the kernel on
@@ -547,6 +567,7 @@ def main() -> int:
         for label, kw in (("generic_rows_kernel", dict(skip_zero=False, schedule=1)),
                           ("packed_rows_per_lane", dict(skip_zero=False, schedule=2)),
                           ("ranked_doubles", dict(skip_zero=False, schedule=3)),
+                          ("pair_space_every_term", dict(skip_zero=False, schedule=6)),
                           ("skip_zero_autotuned", dict(skip_zero=True, schedule=0))):
             Hx = mcpi_amd.Hamiltonian(b, Ho, Hint, **kw)
             for _ in range(2):
@@ -554,6 +575,10 @@ def main() -> int:
             others[label] = {"ms_per_hv": round(Hx.apply_timed(dC, dO, reps), 4), "schedule": Hx.schedule}
             if Hx.schedule == 4:
                 others[label]["pattern_table_bytes"] = b.pattern_bytes()
+            if Hx.schedule == 6:
+                # no map is read: the pair product's FP64 flops (Gauss form,
+                # three real products per complex one) are its model
+                others[label]["pair_product_flops"] = pair_product_flops(N_, M_)
             agree = max(agree, float(np.linalg.norm(dO.numpy() - gpu_out) / np.linalg.norm(gpu_out)))
 
     # one- and two-body densities of the same vector on the same basis

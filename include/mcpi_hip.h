@@ -281,7 +281,15 @@ mcpi_status mcpi_hip_ham_set_skip_zero(mcpi_ham h, int on);
  * short, or on a stream being captured before the tables exist),
  * 5 = every term, with the double jumps of each block of 256 rows
  * compacted into full wavefronts before their targets are read from the
- * wave-packed tables (full maps; falls back to 2 otherwise) */
+ * wave-packed tables (full maps; falls back to 2 otherwise),
+ * 6 = every term through the pair space: the amplitudes <u|a_q a_l|C> and
+ * <u|a_l|C> of the bases with N-2 and N-1 particles are gathered and
+ * multiplied by the Hamiltonian's pair matrix and one-body matrix, and each
+ * row gathers its M(M+1)/2 + M products back (any storage mode, reads no
+ * map; 3 <= M <= 12, bases in the reference order; a workspace of about
+ * 2 M(M+1)/2 NC(N-2, M) + M NC(N-1, M) complex entries, allocated on first
+ * use outside stream capture, is kept with the basis; falls back to 2
+ * otherwise or when device memory is short) */
 mcpi_status mcpi_hip_ham_set_schedule(mcpi_ham h, int schedule);
 /* schedule in use (after autotuning; 0 before the first H.v) */
 mcpi_status mcpi_hip_ham_get_schedule(mcpi_ham h, int *schedule);

@@ -405,7 +405,9 @@ class Hamiltonian:
         specialised ranking double jumps on the fly, 4 specialised reading
         the pattern tables of the zero pattern (zero-skipping), 5 every
         term with each block's double jumps compacted into full wavefronts
-        (full maps)."""
+        (full maps), 6 every term through the pair space (the amplitudes of
+        the bases with N-2 and N-1 particles times the pair and one-body
+        matrices, gathered back per row; 3 <= M <= 12)."""
         _check(lib().mcpi_hip_ham_set_schedule(self.handle, int(schedule)), "schedule")
 
     @property

@@ -24,7 +24,7 @@ BIN = os.path.join(HERE, "bin")
 APPS = ["groundStateLieb"]
 ARCH = os.environ.get("MCPI_OFFLOAD_ARCH", "gfx950")
 
-SOURCES = ["basis.hip", "acth.hip", "acth_fast_a.hip", "acth_fast_b.hip", "acth_fast_c.hip", "acth_comp.hip", "acth_multi_a.hip", "acth_multi_b.hip",
+SOURCES = ["basis.hip", "acth.hip", "acth_fast_a.hip", "acth_fast_b.hip", "acth_fast_c.hip", "acth_comp.hip", "acth_pair.hip", "acth_multi_a.hip", "acth_multi_b.hip",
            "acth_multi_c.hip", "rho.hip", "lanczos.hip", "capi.cpp"]
 
 COMMON = [

@@ -17,6 +17,7 @@
 // a wavefront (they depend only on the orbitals), occupations per lane.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <cmath>
 #include <cstring>
 #include <vector>
@@ -131,9 +132,63 @@ mcpi_status build_coefficients(int M, const cplx *Ho, const cplx *Hint, HamCoefD
     // H.v addresses instance x's tables at a fixed stride from instance 0's)
     nz_e.resize((size_t)(M + std::max(P, 1)) * M2 + 1, 0);
     nz_c.resize(nz_e.size(), cmake(0, 0));
+    // The operator in pair form (schedule 6): every folded term is a
+    // normal-ordered product a*_x a*_y a_z a_w or a*_x a_z.  In the gather
+    // form a row v collects C(w) for
+    //   diagonal      d1[k] v_k(v_k-1) = a*_k a*_k a_k a_k,  d2[k,s] v_k v_s = a*_k a*_s a_s a_k
+    //   k -> l        A0[k,l] a*_k a_l;  with a spectator s, A[k,l,s] v_s a*_k a_l, which is
+    //                 a*_k a*_s a_s a_l (s not k), or a*_k a*_k a_k a_l + a*_k a_l (s = k)
+    //   OT, TT        a*_k a*_k a_q a_l,  a*_k a*_s a_q a_l
+    // so the pair matrix collects the two-body coefficients and h1 the
+    // one-body ones.  Both are uploaded with the other tables.
+    const bool pair = M >= 3 && M <= MCPI_PAIR_MAXM;
+    const int PWS = pair_stride(M);
+    std::vector<cplx> pw(pair ? (size_t)PWS * PWS : 0, cmake(0, 0)), ph1(pair ? 16 * 16 : 0, cmake(0, 0));
+    if (pair) {
+        auto addw = [&](int c1, int c2, int a1, int a2, cplx z) {
+            cplx &w = pw[(size_t)pair_of2(a1, a2) * PWS + pair_of2(c1, c2)];
+            w = cadd(w, z);
+        };
+        auto addh = [&](int k, int l, cplx z) { ph1[(size_t)l * 16 + k] = cadd(ph1[(size_t)l * 16 + k], z); };
+        for (int k = 0; k < M; k++) {
+            addh(k, k, dk[k]);
+            addw(k, k, k, k, d1[k]);
+            for (int s = k + 1; s < M; s++) addw(k, s, k, s, d2[k + M * s]);
+        }
+        for (int k = 0; k < M; k++)
+            for (int l = 0; l < M; l++) {
+                if (l == k) continue;
+                const int kl = k + M * l;
+                addh(k, l, A0[kl]);
+                for (int s = 0; s < M; s++) {
+                    const cplx a = A[(int64_t)kl * M + s];
+                    if (s == k) {
+                        addw(k, k, k, l, a);
+                        addh(k, l, a);
+                    } else {
+                        addw(k, s, s, l, a);
+                    }
+                }
+            }
+        for (int k = 0; k < M; k++)
+            for (int l = 0; l < M; l++)
+                for (int q = 0; q <= l; q++) {
+                    if (q == k || l == k) continue;
+                    addw(k, k, q, l, ot[k * M2 + q + (int64_t)M * l]);
+                }
+        for (int k = 0; k < M; k++)
+            for (int s = k + 1; s < M; s++) {
+                const int p = pair_index(k, s, M);
+                for (int l = 0; l < M; l++)
+                    for (int q = 0; q <= l; q++) {
+                        if (q == k || q == s || l == k || l == s) continue;
+                        addw(k, s, q, l, tt[p * M2 + q + (int64_t)M * l]);
+                    }
+            }
+    }
     // one device block: complex tables first (16-byte aligned), then integers
     size_t nC = dk.size() + d1.size() + d2.size() + A0.size() + A.size() + ot.size() + tt.size() + nz_c.size() +
-                otc.size() + ttc.size();
+                otc.size() + ttc.size() + pw.size() + ph1.size();
     size_t nB = nz1.size() + nzo.size() + nzt.size() + 4 * (m1j.size() + ot_off.size() + tt_off.size() +
                                                            nz_e.size()) + 64;
     size_t bytes = (nC * sizeof(cplx) + nB + 64 + 255) & ~(size_t)255;
@@ -152,6 +207,8 @@ mcpi_status build_coefficients(int M, const cplx *Ho, const cplx *Hint, HamCoefD
     size_t o_nc = put(nz_c.data(), nz_c.size() * sizeof(cplx));
     size_t o_otc = put(otc.data(), otc.size() * sizeof(cplx));
     size_t o_ttc = put(ttc.data(), ttc.size() * sizeof(cplx));
+    const size_t o_pw = pw.empty() ? 0 : put(pw.data(), pw.size() * sizeof(cplx));
+    const size_t o_ph1 = ph1.empty() ? 0 : put(ph1.data(), ph1.size() * sizeof(cplx));
     size_t o_m1 = put(m1j.data(), m1j.size() * 4);
     size_t o_oo = put(ot_off.data(), ot_off.size() * 4);
     size_t o_to = put(tt_off.data(), tt_off.size() * 4);
@@ -173,6 +230,8 @@ mcpi_status build_coefficients(int M, const cplx *Ho, const cplx *Hint, HamCoefD
     c->tt = (cplx *)(dev + o_tt);
     c->otc = (cplx *)(dev + o_otc);
     c->ttc = (cplx *)(dev + o_ttc);
+    c->pw = pair ? (cplx *)(dev + o_pw) : nullptr;
+    c->ph1 = pair ? (cplx *)(dev + o_ph1) : nullptr;
     c->nz_1j = (uint8_t *)(dev + o_n1);
     c->nz_ot = (uint8_t *)(dev + o_no);
     c->nz_tt = (uint8_t *)(dev + o_nt);
@@ -479,6 +538,39 @@ static bool pattern_schedule_ok(mcpi_ham h)
     return h->skip_zero && !h->c.dense && h->basis->maps >= MCPI_MAPS_SINGLE && has_fast_kernel(b.N, b.M);
 }
 
+// the basis's pair-space workspace (schedule 6), allocated when absent and
+// allowed (not under capture, memory free beyond the reserve); nullptr when
+// not available
+static cplx *pair_workspace(mcpi_basis b, bool may_alloc)
+{
+    const size_t need = sizeof(cplx) * pair_workspace_entries(b->d.N, b->d.M);
+    if (b->pair_ws && b->pair_ws_bytes >= need) return (cplx *)b->pair_ws;
+    if (!may_alloc) return nullptr;
+    size_t free_b = 0, total_b = 0;
+    if (hipMemGetInfo(&free_b, &total_b) != hipSuccess || free_b < need + ((size_t)4 << 30)) {
+        (void)hipGetLastError();
+        return nullptr;
+    }
+    void *p = nullptr;
+    if (hipMalloc(&p, std::max<size_t>(need, 16)) != hipSuccess) {
+        (void)hipGetLastError();
+        return nullptr;
+    }
+    // zero once: the rows of the pair amplitudes past the pairs are read
+    // (against zero rows of the pair matrix) and never written
+    if (hipMemset(p, 0, need) != hipSuccess) {
+        (void)hipGetLastError();
+        (void)hipFree(p);
+        return nullptr;
+    }
+    if (b->pair_ws) (void)hipFree(b->pair_ws);
+    b->pair_ws = p;
+    b->pair_ws_bytes = need;
+    return (cplx *)p;
+}
+
+static bool pair_schedule_ok(mcpi_ham h) { return pair_applicable(h->basis->d, h->c); }
+
 // one H.v with schedule `sched`; 4 = specialised kernel reading the pattern
 // tables of the Hamiltonian's zero pattern (built on first use unless the
 // stream is capturing or memory is short; schedule 2 otherwise)
@@ -503,6 +595,15 @@ static mcpi_status launch_sched(mcpi_ham h, int sched, const cplx *C, cplx *out,
         bd.ppoffOT = pt->offOT;
         bd.ppoffTT = pt->offTT;
         return launch_apply_raw(bd, h->basis->maps, h->c, 1, 2, C, out, st);
+    }
+    if (sched == 6) {
+        if (pair_schedule_ok(h)) {
+            hipStreamCaptureStatus cs = hipStreamCaptureStatusNone;
+            MCPI_HIP(hipStreamIsCapturing(st, &cs));
+            cplx *ws = pair_workspace(h->basis, cs == hipStreamCaptureStatusNone);
+            if (ws) return launch_pair(b, h->c, C, out, ws, st);
+        }
+        sched = 2;
     }
     if (sched == 4) {
         const PatternTables *pt = nullptr;
@@ -532,13 +633,14 @@ static mcpi_status autotune(mcpi_ham h, const cplx *C, cplx *out, hipStream_t st
         h->tuned[h->skip_zero ? 1 : 0] = 4;
         return MCPI_OK;
     }
-    int cands[5], n = 0;
+    int cands[6], n = 0;
     cands[n++] = 1;
     if (has_fast_kernel(b.N, b.M)) {
         cands[n++] = 2;
         if (maps == MCPI_MAPS_FULL) cands[n++] = 3;
         if (maps == MCPI_MAPS_FULL && b.packOT && b.packTT) cands[n++] = 5;
         if (pattern_schedule_ok(h) && basis_pattern(h->basis, h->c, true, st)) cands[n++] = 4;
+        if (pair_schedule_ok(h) && pair_workspace(h->basis, true)) cands[n++] = 6;
     }
     int best = cands[n - 1];
     if (n > 1) {

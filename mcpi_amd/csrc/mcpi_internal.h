@@ -166,6 +166,10 @@ struct mcpi_basis_s {
     void *rho_ws = nullptr;
     size_t rho_ws_bytes = 0;
     std::mutex rho_mu;
+    // pair-space H.v workspace (schedule 6: the Y tables of the bases with
+    // N-1 and N-2 particles), allocated on first use and kept
+    void *pair_ws = nullptr;
+    size_t pair_ws_bytes = 0;
     // Sector order (bases created with a conserved charge): rows sorted by
     // the charge sum_j q_j v_j, reference order kept inside a sector.
     // d.perm (device) maps reference index -> row, iperm row -> reference
@@ -208,6 +212,14 @@ struct HamCoefDev {
     int32_t *tt_off; // P+1: same for TT chunk p
     int32_t *nz_e;   // entry index q + l*M of each listed term
     cplx *nz_c;      // its folded coefficient
+    // The same operator through the pair space (schedule 6, 3 <= M <= 12;
+    // null otherwise): H = sum_{k,l} h1[k][l] a*_k a_l
+    //                    + sum_{P,Q} pw[P][Q] a*_{P1} a*_{P2} a_{Q1} a_{Q2}
+    // over unordered orbital pairs P, Q (index p = hi(hi+1)/2 + lo), stored
+    // pw[Q * PWS + P] and ph1[l * 16 + k] (column of an annihilated pair or
+    // orbital contiguous, PWS = M(M+1)/2 rounded up to 16, zero padded)
+    cplx *pw;
+    cplx *ph1;
     int real;        // 1 if every coefficient above has a zero imaginary part
     int dense;       // 1 if at least half of the double-jump terms are non-zero:
                      // walking the lists would cost more than it skips
@@ -237,8 +249,24 @@ __host__ __device__ inline HamCoefDev shift_coef(HamCoefDev c, ptrdiff_t d)
     c.tt_off = sh(c.tt_off);
     c.nz_e = sh(c.nz_e);
     c.nz_c = sh(c.nz_c);
+    if (c.pw) c.pw = sh(c.pw);
+    if (c.ph1) c.ph1 = sh(c.ph1);
     return c;
 }
+
+// pair-space tables of schedule 6: M(M+1)/2 pairs, padded to 16
+__host__ __device__ constexpr int pair_count(int M) { return M * (M + 1) / 2; }
+__host__ __device__ constexpr int pair_stride(int M) { return (pair_count(M) + 15) / 16 * 16; }
+// rows of the pair-amplitude matrix X of the pair-space H.v: the pairs,
+// rounded up to four (the rows past the pairs stay zero)
+__host__ __device__ constexpr int pair_x_rows(int M) { return (pair_count(M) + 3) / 4 * 4; }
+// row stride of the pair-space matrices over nu states: nu rounded up to 16
+__host__ __device__ constexpr int64_t pair_ld(int64_t nu) { return (nu + 15) / 16 * 16; }
+__host__ __device__ inline int pair_of2(int a, int b)
+{
+    return a < b ? b * (b + 1) / 2 + a : a * (a + 1) / 2 + b;
+}
+#define MCPI_PAIR_MAXM 12
 
 struct mcpi_ham_s {
     mcpi_basis basis;
@@ -296,6 +324,13 @@ mcpi_status basis_vectors(mcpi_basis b, cplx **v0, cplx **v1);
 mcpi_status rank_tables_dev(int N, int M, int32_t **B, int32_t **NCmat);
 // w[0 .. n) = a[0 .. n) widened to int64, on stream st
 mcpi_status widen_strides(int64_t n, const int32_t *a, int64_t *w, hipStream_t st);
+// schedule 6 (acth_pair.hip): the pair-space H.v applies (3 <= M <= 12,
+// reference row order, nc < 2^31); workspace entries; launch into the
+// workspace ws of pair_workspace_entries(N, M) complex entries
+bool pair_applicable(const BasisDev &b, const HamCoefDev &c);
+size_t pair_workspace_entries(int N, int M);
+mcpi_status launch_pair(const BasisDev &b, const HamCoefDev &c, const cplx *C, cplx *out, cplx *ws,
+                        hipStream_t st);
 // build the requested maps of a basis whose IF table is in place
 mcpi_status basis_build_parts(mcpi_basis b, int want_map, int want_ot, int want_tt);
 

@@ -36,13 +36,15 @@
 // LDS, so a gather costs two LDS reads and two subtractions.
 //
 // Kernel: a block owns one tile pair (TI, TJ) of 80 x 80 entries of the
-// Gram matrix and loops over tiles of 32 rows of Phi: it unranks the rows,
-// gathers the 32 x 80 (x 2 off the diagonal) entries of Phi into LDS
-// (lanes over consecutive rows of one column: neighbouring targets), then
-// every thread adds 15 (diagonal tile pair: the sub-blocks i <= j of 16 x
-// 16) or 25 sub-block entries with FP64 FMAs.  Each block writes its tile
-// once; a second kernel sums the blocks in a fixed order (deterministic
-// result), and a third expands the Gram matrix into the full rho1 / rho2
+// Gram matrix and loops over tiles of U rows of Phi (32 for rho2, 128 for
+// rho1, whose Phi has at most 32 columns): it unranks the rows (one thread
+// per row), gathers the U x 16 NG (x 2 off the diagonal) entries of Phi
+// into LDS (lanes over consecutive rows of one column: neighbouring
+// targets; GB loads in flight per thread), then every thread adds its
+// entry of each 16 x 16 sub-block (the sub-blocks i <= j of a diagonal
+// tile pair, all of them off it) with FP64 FMAs.  Each block writes its
+// tile once; two fixed-order stages sum the blocks (deterministic result),
+// and a last kernel expands the Gram matrix into the full rho1 / rho2
 // arrays on the device.  No map is read, so the densities run on every
 // storage mode, and the vectors are in the reference order.
 #include <hip/hip_runtime.h>

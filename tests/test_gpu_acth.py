@@ -25,7 +25,9 @@ def schedules(M, maps):
     4 specialised with the pattern tables of the zero pattern (zero-skipping
     on a sparse Hamiltonian with Map stored; otherwise it runs as 2);
     5 every term with each block's double jumps compacted into full
-    wavefronts (full maps; otherwise it runs as 2)."""
+    wavefronts (full maps; otherwise it runs as 2); 6 every term through
+    the pair space (two gathers around a dense product with the pair
+    matrix, any storage mode, M <= 12; otherwise it runs as 2)."""
     out = [1]
     if 3 <= M <= 13:
         out.append(2)
@@ -34,6 +36,7 @@ def schedules(M, maps):
         if maps != "none":
             out.append(4)
         out.append(5)
+        out.append(6)
     return out
 
 
@@ -245,7 +248,7 @@ def test_n12_m12_generic_sampled_rows(oracle_lib):
     out = H(C)
     rows = np.concatenate([[0, b.nc - 1], rng.integers(0, b.nc, 1500)])
     assert_close(out[rows], oracle.acth_hash_rows(N, M, rows, C, Ho, Hint), 1e-12, "rows")
-    for sched in (1, 2, 3, 5):
+    for sched in (1, 2, 3, 5, 6):
         H2 = mcpi_amd.Hamiltonian(b, Ho, Hint, skip_zero=False, schedule=sched)
         assert_close(H2(C), out, 1e-12, f"schedule {sched}")
 
@@ -264,6 +267,6 @@ def test_n16_m12_single_map_sampled_rows(oracle_lib):
     rows = np.concatenate([[0, b.nc - 1], rng.integers(0, b.nc, 3000)])
     assert_close(out[rows], oracle.acth_hash_rows(N, M, rows, C, Ho, Hint), 1e-12, "rows")
     _hermitian_check(H, b.nc, 9)
-    for sched in (2, 4):
+    for sched in (2, 4, 6):
         H2 = mcpi_amd.Hamiltonian(b, Ho, Hint, skip_zero=True, schedule=sched)
         assert_close(H2(C), out, 1e-12, f"schedule {sched}")
