@@ -538,14 +538,16 @@ static bool pattern_schedule_ok(mcpi_ham h)
     return h->skip_zero && !h->c.dense && h->basis->maps >= MCPI_MAPS_SINGLE && has_fast_kernel(b.N, b.M);
 }
 
-// the basis's pair-space workspace (schedule 6), allocated when absent and
-// allowed (not under capture, memory free beyond the reserve); nullptr when
-// not available
-static cplx *pair_workspace(mcpi_basis b, bool may_alloc)
+// the basis's pair-space workspace (schedule 6) for stream st, allocated
+// when absent and allowed (not under capture, memory free beyond the
+// reserve); nullptr when not available
+static cplx *pair_workspace(mcpi_basis b, hipStream_t st, bool may_alloc)
 {
-    const size_t need = sizeof(cplx) * pair_workspace_entries(b->d.N, b->d.M);
-    if (b->pair_ws && b->pair_ws_bytes >= need) return (cplx *)b->pair_ws;
+    std::lock_guard<std::mutex> lock(b->pair_mu);
+    for (const auto &w : b->pair_ws)
+        if (w.st == st) return (cplx *)w.p;
     if (!may_alloc) return nullptr;
+    const size_t need = sizeof(cplx) * pair_workspace_entries(b->d.N, b->d.M);
     size_t free_b = 0, total_b = 0;
     if (hipMemGetInfo(&free_b, &total_b) != hipSuccess || free_b < need + ((size_t)4 << 30)) {
         (void)hipGetLastError();
@@ -563,9 +565,7 @@ static cplx *pair_workspace(mcpi_basis b, bool may_alloc)
         (void)hipFree(p);
         return nullptr;
     }
-    if (b->pair_ws) (void)hipFree(b->pair_ws);
-    b->pair_ws = p;
-    b->pair_ws_bytes = need;
+    b->pair_ws.push_back({st, p});
     return (cplx *)p;
 }
 
@@ -600,7 +600,7 @@ static mcpi_status launch_sched(mcpi_ham h, int sched, const cplx *C, cplx *out,
         if (pair_schedule_ok(h)) {
             hipStreamCaptureStatus cs = hipStreamCaptureStatusNone;
             MCPI_HIP(hipStreamIsCapturing(st, &cs));
-            cplx *ws = pair_workspace(h->basis, cs == hipStreamCaptureStatusNone);
+            cplx *ws = pair_workspace(h->basis, st, cs == hipStreamCaptureStatusNone);
             if (ws) return launch_pair(b, h->c, C, out, ws, st);
         }
         sched = 2;
@@ -640,7 +640,7 @@ static mcpi_status autotune(mcpi_ham h, const cplx *C, cplx *out, hipStream_t st
         if (maps == MCPI_MAPS_FULL) cands[n++] = 3;
         if (maps == MCPI_MAPS_FULL && b.packOT && b.packTT) cands[n++] = 5;
         if (pattern_schedule_ok(h) && basis_pattern(h->basis, h->c, true, st)) cands[n++] = 4;
-        if (pair_schedule_ok(h) && pair_workspace(h->basis, true)) cands[n++] = 6;
+        if (pair_schedule_ok(h) && pair_workspace(h->basis, st, true)) cands[n++] = 6;
     }
     int best = cands[n - 1];
     if (n > 1) {

@@ -570,7 +570,7 @@ static void free_basis(mcpi_basis b)
             if (p) (void)hipFree(p);
     }
     if (b->rho_ws) (void)hipFree(b->rho_ws);
-    if (b->pair_ws) (void)hipFree(b->pair_ws);
+    for (auto &w : b->pair_ws) (void)hipFree(w.p);
     void *sec[] = {(void *)b->d.perm, b->iperm, b->IF_ref, b->vt[0], b->vt[1]};
     for (void *p : sec)
         if (p) (void)hipFree(p);

@@ -166,10 +166,16 @@ struct mcpi_basis_s {
     void *rho_ws = nullptr;
     size_t rho_ws_bytes = 0;
     std::mutex rho_mu;
-    // pair-space H.v workspace (schedule 6: the Y tables of the bases with
-    // N-1 and N-2 particles), allocated on first use and kept
-    void *pair_ws = nullptr;
-    size_t pair_ws_bytes = 0;
+    // pair-space H.v workspaces (schedule 6: the pair amplitudes and the Y
+    // tables of the bases with N-1 and N-2 particles), one per stream that
+    // ran the schedule (so that H.v calls on different streams never share
+    // one), allocated on first use and kept
+    struct PairWs {
+        hipStream_t st;
+        void *p;
+    };
+    std::vector<PairWs> pair_ws;
+    std::mutex pair_mu;   // guards pair_ws
     // Sector order (bases created with a conserved charge): rows sorted by
     // the charge sum_j q_j v_j, reference order kept inside a sector.
     // d.perm (device) maps reference index -> row, iperm row -> reference
