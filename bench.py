@@ -280,6 +280,61 @@ def reference_shaped_calls(reps: int = 20):
             "max_rel_diff_vs_device_api": same, "first_vs_repeated_max_diff": float(np.abs(first - out).max())}
 
 
+def random_hamiltonian(M: int, seed: int, sigma: float = 0.1):
+    """Random Hermitian Ho and a random bosonic Hint, symmetric under k <-> l
+    and q <-> s and with Hint[k,l,q,s] = conj(Hint[q,s,k,l]); entries ~ N(0,
+    sigma) (re and im) before the symmetrization.  (With this symmetry the
+    reference's rules read the operator 1/2 sum Hint a*_k a*_l a_q a_s, so
+    E0 = sum Ho rho1 + 1/2 sum Hint rho2 holds at the eigenvector.)"""
+    rng = np.random.default_rng(seed)
+    A = sigma * (rng.standard_normal((M, M)) + 1j * rng.standard_normal((M, M)))
+    Ho = 0.5 * (A + A.conj().T)
+    B = sigma * (rng.standard_normal((M,) * 4) + 1j * rng.standard_normal((M,) * 4))
+    B = 0.5 * (B + B.transpose(1, 0, 2, 3))
+    B = 0.5 * (B + B.transpose(0, 1, 3, 2))
+    Hint = 0.5 * (B + np.conj(B.transpose(2, 3, 0, 1)))
+    return Ho, Hint
+
+
+def adjoint_n10_m12(lm: int = 300, seed: int = 1012):
+    """The adjoint configuration: N=10, M=12 (nc = 352,716), random Hermitian
+    Ho and bosonic random Hint (entries ~ N(0, 0.1)), full maps;
+    Lanczos ground state (lm steps, full reorthogonalization) from a seeded
+    random vector, then the VJP (dE0/dHo = rho1, dE0/dHint = rho2 / 2).  The
+    Hamiltonian is dense, so the autotuner's H.v is the pair-space one."""
+    import mcpi_amd
+    N, M = 10, 12
+    t = time.perf_counter()
+    b = mcpi_amd.Basis(N, M, "full")
+    build_s = time.perf_counter() - t
+    Ho, Hint = random_hamiltonian(M, seed)
+    rng = np.random.default_rng(seed + 1)
+    C0 = rng.standard_normal(b.nc) + 1j * rng.standard_normal(b.nc)
+    dC0 = mcpi_amd.DeviceArray.from_numpy(C0 / np.linalg.norm(C0))
+    H = mcpi_amd.Hamiltonian(b, Ho, Hint)
+    dO = mcpi_amd.DeviceArray(b.nc)
+    H.apply(dC0, dO)                            # schedule tuning
+    ms_hv = H.apply_timed(dC0, dO, 20)
+    H.ground(lm, dC0)                           # untimed first call (graph capture, allocations)
+    mcpi_amd.synchronize()
+    t = time.perf_counter()
+    E, psi, steps = H.ground(lm, dC0)
+    t_ground = time.perf_counter() - t
+    t = time.perf_counter()
+    g1, g2 = b.vjp(psi)
+    t_vjp = time.perf_counter() - t
+    v = psi.numpy()
+    res = float(np.linalg.norm(H(v) - E * v))
+    out = {"workload": "N=10, M=12, random Hermitian Ho / bosonic symmetric Hint (sigma 0.1), full maps",
+           "nc": b.nc, "basis_build_s": round(build_s, 3), "schedule": H.schedule, "ms_per_hv": round(ms_hv, 4),
+           "lm": lm, "steps": steps, "E0": E, "ground_s": round(t_ground, 4), "vjp_s": round(t_vjp, 4),
+           "residual_norm": res,
+           "energy_identity_residual": float(abs(np.real(np.sum(g1 * Ho) + np.sum(g2 * Hint)) - E))}
+    H.close()
+    b.close()
+    return out
+
+
 def batched_lanczos(n_inst: int = 64, lm: int = 120, seed: int = 64120, serial: bool = True,
                     cpu_sample: bool = True):
     """64 Lieb-Liniger instances, (N, M) drawn from the seed over (6,9),
@@ -494,6 +549,8 @@ def main() -> int:
     ap.add_argument("--batched-only", action="store_true",
                     help="only run the batched Lanczos call, without the per-instance solves "
                          "(profiling runs)")
+    ap.add_argument("--adjoint-only", action="store_true",
+                    help="only run the N=10, M=12 ground state and adjoint of a dense random Hamiltonian")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only run three 1 GiB device-to-device copies with the library's copy kernel "
                          "(reference reading for the FETCH_SIZE / WRITE_SIZE counters)")
@@ -525,6 +582,9 @@ def main() -> int:
         return 0
     if args.batched_only:   # the batched call alone (no per-instance solves)
         print(json.dumps({"batched_lanczos": batched_lanczos(serial=False)}))
+        return 0
+    if args.adjoint_only:
+        print(json.dumps({"adjoint_n10_m12": adjoint_n10_m12()}))
         return 0
 
     name, cus, mem = mcpi_amd.device_info()
@@ -604,6 +664,9 @@ def main() -> int:
     if not (args.no_batched or args.profile):
         del dO
         batched = batched_lanczos(cpu_sample=not args.no_cpu)
+    adjoint = None
+    if not args.profile:
+        adjoint = adjoint_n10_m12()
 
     cpu = None
     if not (args.no_cpu or args.profile):
@@ -653,6 +716,7 @@ def main() -> int:
             "densities_n12_m12": dens,
             "reference_shaped_applyHconf_omp_n10_m12": refcalls,
             "batched_lanczos": batched,
+            "adjoint_n10_m12": adjoint,
             "large_basis_n16_m12": large,
             "max_rel_diff_between_schedules": agree,
             "speedup_vs_cpu": (round(cpu["ms_per_hv"] / ms_step, 1) if cpu else None),

@@ -336,7 +336,14 @@ mcpi_status mcpi_hip_applyH_timed_each(mcpi_ham h, const mcpi_cplx *C_dev,
 mcpi_status mcpi_hip_rho1(mcpi_basis b, const mcpi_cplx *C_dev, mcpi_cplx *rho1);
 mcpi_status mcpi_hip_rho2(mcpi_basis b, const mcpi_cplx *C_dev, mcpi_cplx *rho2);
 /* Hellmann-Feynman adjoint at a normalized eigenvector C: dE/dHo = rho1,
- * dE/dHint = rho2 / 2 (host outputs, flat layouts above) */
+ * dE/dHint = rho2 / 2 (host outputs, flat layouts above).  These are the
+ * gradients over Hermitian Ho and bosonic Hint (symmetric under k <-> l and
+ * under q <-> s, Hint[k,l,q,s] = conj(Hint[q,s,k,l])), the inputs for which
+ * the reference's rules apply 1/2 sum Hint a*_k a*_l a_q a_s.  The rules
+ * read particular entries of Hint and weight them as if it had that
+ * symmetry, so for a Hint with only the weaker symmetry Hint[k,l,q,s] =
+ * Hint[l,k,s,q] they apply another operator (the H.v here follows them) and
+ * E != sum Ho rho1 + 1/2 sum Hint rho2 in general. */
 mcpi_status mcpi_hip_vjp(mcpi_basis b, const mcpi_cplx *C_dev, mcpi_cplx *dE_dHo,
                          mcpi_cplx *dE_dHint);
 /* No counterpart in the reference.  For callers that cannot allocate device
