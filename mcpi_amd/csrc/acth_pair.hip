@@ -1,0 +1,607 @@
+// H.v through the pair space (schedule 6): every term of the reference's
+// operator (hamiltonianMatrix.h:1431-1970, the folded rules of acth.hip),
+// evaluated as two gathers around a dense product instead of one gather per
+// term.
+//
+// In pair form (build_coefficients) the operator is
+//   H = sum_{k,l} h1[k][l] a*_k a_l + sum_{P,Q} W[P][Q] a*_{P1} a*_{P2} a_{Q1} a_{Q2}
+// over unordered orbital pairs.  Inserting the identity over the basis with
+// N-2 (N-1) particles after the creators,
+//   (H C)(v) = sum_P <v|a*_{P1} a*_{P2}|u> Y2_u(P) + sum_k <v|a*_k|u'> Y1_u'(k),
+//   Y2_u(P)  = sum_Q W[P][Q] X[Q][u],  X[Q][u] = <u|a_{Q1} a_{Q2}|C>,  u  = v - e_{P1} - e_{P2}
+//   Y1_u'(k) = sum_l h1[k][l] <u'|a_l|C>,                             u' = v - e_k
+// and <u|a_q a_l|C> = sqrt((u_q + 1)(u_l + 1 + [q = l])) C[rank(u + e_q + e_l)]
+// is the pair amplitude of the densities' Gram matrix (rho.hip).  At N = M =
+// 12 a row has up to 1,636 double-jump terms; here it costs 78 + 12 output
+// gathers, and the 352,716 rows of 10 particles cost 78 gathers and a
+// 78 x 78 complex matrix-vector product each, on the FP64 MFMA.
+//
+// k_pair_y1<M, U>: Y1 for tiles of U rows of the basis with N-1 particles
+// (amplitudes gathered into LDS, then h1 times them with h1 in LDS).
+// k_pair_amp<M>: X, gathered.  k_pair_mul<M>: Y2 = W X on the MFMA.
+// k_pair_out<M>: one thread per row v of the basis ranks the v - e_P and
+// v - e_k in the smaller bases from its prefix-count tables and sums the
+// factors times Y.  Y1, Y2 and X are stored column-major (Y[col * ld + u])
+// so that every pass reads and writes them coalesced.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+
+#include "mcpi_internal.h"
+
+namespace mcpi {
+
+namespace {
+
+constexpr int PT = 256;   // threads per block
+
+struct PairArgs {
+    int N;                // particles of the basis
+    int64_t nc;           // its states
+    int64_t nu;           // states of the basis with N - 2 (or N - 1) particles
+    int64_t nut;          // tiles of U rows
+    const int32_t *B;     // rank table B[j*(N+1) + p]
+    const cplx *C;
+    const cplx *W;        // pair matrix pw[Q * PWS + P], or h1 as ph1[l * 16 + k]
+    cplx *Y;              // Y[col * ld + u]
+    int64_t ld;           // row stride of Y (and of the pair amplitudes X), >= nu
+};
+
+__device__ inline cplx ldcp(const cplx *p)
+{
+    double2 t = *reinterpret_cast<const double2 *>(p);
+    return cmake(t.x, t.y);
+}
+
+template <int M, int U>
+struct PairLds {
+    static constexpr int RS = M + 1;   // LDS row stride of the tile (odd)
+    static size_t bytes(int N)
+    {
+        size_t o = sizeof(cplx) * ((size_t)M * 16 + (size_t)U * RS);
+        o += sizeof(int64_t) * ((size_t)U + (size_t)U * M);
+        o += sizeof(double) * (N + 3) + sizeof(int32_t) * M * (N + 1) + (size_t)U * (M + 1);
+        return (o + 15) / 16 * 16;
+    }
+};
+
+// Y1[k][u] = sum_l h1[k][l] <u|a_l|C> over the basis with N-1 particles
+template <int M, int U>
+__global__ __launch_bounds__(PT) void k_pair_y1(PairArgs a)
+{
+    using L = PairLds<M, U>;
+    constexpr int RS = L::RS, UR = U / 16;
+    extern __shared__ __align__(16) unsigned char lds[];
+    cplx *Wl = (cplx *)lds;                                   // [M][16]: h1[k][l] at l * 16 + k
+    cplx *phi = Wl + M * 16;                                   // [U][RS]
+    int64_t *tt = (int64_t *)(phi + U * RS);                  // [U]
+    int64_t *dq = tt + U;                                      // [U][M]
+    double *sq = (double *)(dq + U * M);                       // [N + 3]
+    int32_t *Bt = (int32_t *)(sq + a.N + 3);                   // [M][N + 1]
+    uint8_t *pp = (uint8_t *)(Bt + M * (a.N + 1));            // [U][M + 1]
+
+    const int t = threadIdx.x;
+    const int N = a.N, NB = N + 1, Np = N - 1;
+    for (int x = t; x < M * 16; x += PT) Wl[x] = ldcp(a.W + x);
+    for (int x = t; x < M * NB; x += PT) Bt[x] = a.B[x];
+    for (int x = t; x < N + 3; x += PT) sq[x] = sqrt((double)x);
+    const int pg = t & 15, ug = t >> 4;
+
+    for (int64_t tile = blockIdx.x; tile < a.nut; tile += gridDim.x) {
+        __syncthreads();   // the previous tile's product has read phi and the row tables
+        const int64_t u0 = tile * U;
+        if (t < U) {
+            const int64_t u = u0 + t;
+            uint8_t *P = pp + t * (M + 1);
+            if (u < a.nu) {
+                // greedy unranking, one step per iteration (lanes stay together)
+                int64_t R = a.nu - 1 - u;
+                int pj = Np, j = M - 1;
+                P[M] = (uint8_t)Np;
+                while (j >= 1) {
+                    const int64_t bj = Bt[j * NB + pj];
+                    if (bj > R) {
+                        pj--;
+                    } else {
+                        P[j] = (uint8_t)pj;
+                        R -= bj;
+                        j--;
+                    }
+                }
+                P[0] = 0;
+                int64_t s0 = 0, s1 = 0;
+                int64_t *Dq = dq + t * M;
+                Dq[0] = 0;
+                for (int j2 = 1; j2 < M; j2++) {
+                    const int b = j2 * NB + P[j2];
+                    s0 += Bt[b];
+                    s1 += Bt[b + 1];
+                    Dq[j2] = s0 - s1;
+                }
+                tt[t] = a.nc - 1 - s1;
+            } else {
+                tt[t] = -1;
+            }
+        }
+        __syncthreads();
+        // amplitudes of the tile: lanes over consecutive rows of one column,
+        // five loads in flight per thread
+        constexpr int NE = U * M, GBP = 5;
+        for (int e0 = t; e0 < NE; e0 += GBP * PT) {
+            int64_t tg[GBP];
+            double f[GBP];
+#pragma unroll
+            for (int k = 0; k < GBP; k++) {
+                const int e = e0 + k * PT;
+                const int col = e / U, ul = e % U;
+                const int64_t base = e < NE ? tt[ul] : -1;
+                tg[k] = -1;
+                f[k] = 0;
+                if (base >= 0) {
+                    const uint8_t *P = pp + ul * (M + 1);
+                    tg[k] = base - dq[ul * M + col];
+                    f[k] = sq[P[col + 1] - P[col] + 1];
+                }
+            }
+            cplx v[GBP];
+#pragma unroll
+            for (int k = 0; k < GBP; k++) v[k] = tg[k] >= 0 ? ldcp(a.C + tg[k]) : cmake(0, 0);
+#pragma unroll
+            for (int k = 0; k < GBP; k++) {
+                const int e = e0 + k * PT;
+                if (e < NE) phi[(e % U) * RS + e / U] = cscale(v[k], f[k]);
+            }
+        }
+        __syncthreads();
+        // thread (pg, ug) forms column pg (an orbital k < M) of the rows
+        // ug + 16 j
+        cplx acc[UR];
+#pragma unroll
+        for (int j = 0; j < UR; j++) acc[j] = cmake(0, 0);
+#pragma unroll 2
+        for (int l = 0; l < M; l++) {
+            const cplx w = Wl[l * 16 + pg];
+            cplx x[UR];
+#pragma unroll
+            for (int j = 0; j < UR; j++) x[j] = phi[(ug + 16 * j) * RS + l];
+#pragma unroll
+            for (int j = 0; j < UR; j++) acc[j] = cfma(w, x[j], acc[j]);
+        }
+        if (pg < M) {
+#pragma unroll
+            for (int j = 0; j < UR; j++) {
+                const int64_t u = u0 + ug + 16 * j;
+                if (u < a.nu) {
+                    double2 o;
+                    o.x = acc[j].re;
+                    o.y = acc[j].im;
+                    *reinterpret_cast<double2 *>(a.Y + (int64_t)pg * a.ld + u) = o;
+                }
+            }
+        }
+    }
+}
+
+// The pair amplitudes X[Q][u] = <u|a_{Q1} a_{Q2}|C> of the basis with N-2
+// particles, column-major (X[Q * nu + u]): one thread per row of a 64-row
+// tile unranks it, then the block's lanes run over consecutive rows of one
+// column at a time (neighbouring targets, coalesced stores), five loads in
+// flight per thread.  Small LDS, so many blocks share a CU and hide each
+// other's unranking.
+template <int M>
+struct PairAmpLds {
+    static constexpr int U = 64;
+    static size_t bytes(int N)
+    {
+        size_t o = sizeof(int64_t) * ((size_t)U + (size_t)U * M * 2);
+        o += sizeof(double) * (N + 3) + sizeof(int32_t) * M * (N + 1) + (size_t)U * (M + 1);
+        return (o + 15) / 16 * 16;
+    }
+};
+
+template <int M>
+__global__ __launch_bounds__(PT) void k_pair_amp(PairArgs a)
+{
+    constexpr int NP = pair_count(M), U = PairAmpLds<M>::U;
+    extern __shared__ __align__(16) unsigned char lds[];
+    int64_t *tt = (int64_t *)lds;                              // [U]
+    int64_t *dq = tt + U;                                      // [U][M]
+    int64_t *dl = dq + U * M;                                  // [U][M]
+    double *sq = (double *)(dl + U * M);                       // [N + 3]
+    int32_t *Bt = (int32_t *)(sq + a.N + 3);                   // [M][N + 1]
+    uint8_t *pp = (uint8_t *)(Bt + M * (a.N + 1));            // [U][M + 1]
+    const int t = threadIdx.x;
+    const int N = a.N, NB = N + 1, Np = N - 2;
+    for (int x = t; x < M * NB; x += PT) Bt[x] = a.B[x];
+    for (int x = t; x < N + 3; x += PT) sq[x] = sqrt((double)x);
+    for (int64_t tile = blockIdx.x; tile < a.nut; tile += gridDim.x) {
+        __syncthreads();
+        const int64_t u0 = tile * U;
+        if (t < U) {
+            const int64_t u = u0 + t;
+            uint8_t *P = pp + t * (M + 1);
+            if (u < a.nu) {
+                int64_t R = a.nu - 1 - u;
+                int pj = Np, j = M - 1;
+                P[M] = (uint8_t)Np;
+                while (j >= 1) {
+                    const int64_t bj = Bt[j * NB + pj];
+                    if (bj > R) {
+                        pj--;
+                    } else {
+                        P[j] = (uint8_t)pj;
+                        R -= bj;
+                        j--;
+                    }
+                }
+                P[0] = 0;
+                int64_t s0 = 0, s1 = 0, s2 = 0;
+                int64_t *Dq = dq + t * M, *Dl = dl + t * M;
+                Dq[0] = 0;
+                Dl[0] = 0;
+                for (int j2 = 1; j2 < M; j2++) {
+                    const int b = j2 * NB + P[j2];
+                    s0 += Bt[b];
+                    s1 += Bt[b + 1];
+                    s2 += Bt[b + 2];
+                    Dq[j2] = s0 - s1;
+                    Dl[j2] = s1 - s2;
+                }
+                tt[t] = a.nc - 1 - s2;
+            } else {
+                tt[t] = -1;
+            }
+        }
+        __syncthreads();
+        constexpr int NE = U * NP, GBP = 5;
+        for (int e0 = t; e0 < NE; e0 += GBP * PT) {
+            int64_t tg[GBP];
+            double f[GBP];
+#pragma unroll
+            for (int k = 0; k < GBP; k++) {
+                const int e = e0 + k * PT;
+                const int col = e / U, ul = e % U;
+                const int64_t base = e < NE ? tt[ul] : -1;
+                tg[k] = -1;
+                f[k] = 0;
+                if (base >= 0) {
+                    const uint8_t *P = pp + ul * (M + 1);
+                    int l = 0;
+                    while ((l + 1) * (l + 2) / 2 <= col) l++;
+                    const int q = col - l * (l + 1) / 2;
+                    const int uq = P[q + 1] - P[q], ulo = P[l + 1] - P[l];
+                    tg[k] = base - dq[ul * M + q] - dl[ul * M + l];
+                    f[k] = sq[uq + 1] * sq[ulo + 1 + (q == l)];
+                }
+            }
+            cplx v[GBP];
+#pragma unroll
+            for (int k = 0; k < GBP; k++) v[k] = tg[k] >= 0 ? ldcp(a.C + tg[k]) : cmake(0, 0);
+#pragma unroll
+            for (int k = 0; k < GBP; k++) {
+                if (tg[k] < 0) continue;
+                const int e = e0 + k * PT;
+                const cplx z = cscale(v[k], f[k]);
+                double2 o;
+                o.x = z.re;
+                o.y = z.im;
+                *reinterpret_cast<double2 *>(a.Y + (int64_t)(e / U) * a.ld + u0 + e % U) = o;
+            }
+        }
+    }
+}
+
+// Y2 = W X through FP64 MFMA (v_mfma_f64_16x16x4f64: lane i + 16k holds
+// A[i][k] and B[k][i]; the result D[g + 4r][j] sits in register r of lane
+// j + 16g), three real products per complex one (Gauss): T1 = (Xr + Xi)
+// Wr, T2 = Xr (Wi - Wr), T3 = Xi (Wr + Wi), then Re = T1 - T3 and Im = T1 +
+// T2.  The block stages W once in LDS as B fragments in the order the lanes
+// read them: Wa[s][g][lane] = (Wr, Wi - Wr) and Wb[s][g][lane] = Wr + Wi of
+// W[4s + k][16g + j], KS = ceil(NP/4) steps of four Q times NG groups of
+// 16 pair columns P, zero past NP; conflict-free reads.
+// Each wavefront takes 16-row blocks of X on its own and keeps the 16 x NP
+// result block in accumulators: per step one A fragment (four Q rows of 16
+// consecutive u, coalesced; loaded four steps ahead, across row blocks)
+// meets the NG fragments of W, three MFMAs each.
+typedef double mcpi_v4d __attribute__((ext_vector_type(4)));
+
+constexpr int PAIR_MUL_WAVES = 8;
+
+template <int M>
+struct PairMulLds {
+    static constexpr int NP = pair_count(M), KS = (NP + 3) / 4, NG = (NP + 15) / 16;
+    static constexpr size_t bytes = (sizeof(double2) + sizeof(double)) * NG * KS * 64;
+};
+
+template <int M>
+__global__ __launch_bounds__(64 * PAIR_MUL_WAVES) void k_pair_mul(PairArgs a, const cplx *X)
+{
+    using L = PairMulLds<M>;
+    constexpr int NP = L::NP, KS = L::KS, NG = L::NG, PWS = pair_stride(M);
+    extern __shared__ __align__(16) unsigned char lds[];
+    double2 *wa = (double2 *)lds;                 // [KS][NG][64]
+    double *wb = (double *)(wa + NG * KS * 64);   // [KS][NG][64]
+    for (int x = threadIdx.x; x < NG * KS * 64; x += blockDim.x) {
+        const int s = x / (NG * 64), g = (x / 64) % NG, ln = x % 64;
+        const int P = 16 * g + (ln & 15), Q = 4 * s + (ln >> 4);
+        double2 v = {0, 0};
+        double vb = 0;
+        if (P < NP && Q < NP) {
+            const cplx w = ldcp(a.W + (int64_t)Q * PWS + P);
+            v.x = w.re;
+            v.y = w.im - w.re;
+            vb = w.re + w.im;
+        }
+        wa[x] = v;
+        wb[x] = vb;
+    }
+    __syncthreads();
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int jl = lane & 15, kl = lane >> 4;
+    const int64_t nrb = (a.nu + 15) / 16, stride = (int64_t)gridDim.x * PAIR_MUL_WAVES;
+    // (the launch gives ld >= 16 nrb)
+    // A fragments: lane (j, k) of step s of row block rb reads X[4s + k][16
+    // rb + j].  X has 4 KS rows of ld >= 16 nrb entries, zero past NP rows
+    // and nu columns, and Y has 16 NG rows of ld entries, so that neither
+    // loads nor stores need a bound; a block past the last is read as the
+    // last.
+    const int64_t rowstep = 4 * a.ld;
+    auto lanep = [&](int64_t rb) { return X + (int64_t)kl * a.ld + (rb < nrb ? rb : nrb - 1) * 16 + jl; };
+    // They are loaded four steps before their use, across row blocks: the
+    // steps of a block are counted up to KS4 = KS rounded up to four (the
+    // steps past KS load step KS - 1 again and skip their MFMAs), so that
+    // step t of the ring always sits in register t % 4.
+    constexpr int KS4 = (KS + 3) & ~3;
+    auto ldx = [&](const cplx *p, int s) { return ldcp(p + (int64_t)(s < KS ? s : KS - 1) * rowstep); };
+    int64_t rb = (int64_t)blockIdx.x * PAIR_MUL_WAVES + wave;
+    const cplx *pc = lanep(rb);
+    cplx x0 = ldx(pc, 0);
+    __builtin_amdgcn_sched_barrier(0);
+    cplx x1 = ldx(pc, 1);
+    __builtin_amdgcn_sched_barrier(0);
+    cplx x2 = ldx(pc, 2);
+    __builtin_amdgcn_sched_barrier(0);
+    cplx x3 = ldx(pc, 3);
+    __builtin_amdgcn_sched_barrier(0);
+    // one row block: KS4 steps in straight-line code, then the stores
+    auto block = [&](int64_t rb, const cplx *pn) {
+        auto ldnext = [&](int t) { return t < KS4 ? ldx(pc, t) : ldx(pn, t - KS4); };
+        mcpi_v4d acc[3 * NG];
+#pragma unroll
+        for (int g = 0; g < 3 * NG; g++) acc[g] = mcpi_v4d{0, 0, 0, 0};
+        auto step = [&](int s, const cplx xc) {
+            if (s >= KS) return;
+            const double xs = xc.re + xc.im;
+#pragma unroll
+            for (int g = 0; g < NG; g++) {
+                const double2 w = wa[(s * NG + g) * 64 + lane];
+                const double w3 = wb[(s * NG + g) * 64 + lane];
+                acc[3 * g] = __builtin_amdgcn_mfma_f64_16x16x4f64(xs, w.x, acc[3 * g], 0, 0, 0);
+                acc[3 * g + 1] = __builtin_amdgcn_mfma_f64_16x16x4f64(xc.re, w.y, acc[3 * g + 1], 0, 0, 0);
+                acc[3 * g + 2] = __builtin_amdgcn_mfma_f64_16x16x4f64(xc.im, w3, acc[3 * g + 2], 0, 0, 0);
+            }
+            // LDS reads of one group ahead of its three MFMAs, in turn
+#pragma unroll
+            for (int g = 0; g < NG; g++) {
+                __builtin_amdgcn_sched_group_barrier(0x100, 2, 0);
+                __builtin_amdgcn_sched_group_barrier(0x008, 3, 0);
+            }
+        };
+#pragma unroll
+        for (int s = 0; s < KS4; s += 4) {
+            step(s, x0);
+            __builtin_amdgcn_sched_barrier(0);
+            x0 = ldnext(s + 4);
+            step(s + 1, x1);
+            __builtin_amdgcn_sched_barrier(0);
+            x1 = ldnext(s + 5);
+            step(s + 2, x2);
+            __builtin_amdgcn_sched_barrier(0);
+            x2 = ldnext(s + 6);
+            step(s + 3, x3);
+            __builtin_amdgcn_sched_barrier(0);
+            x3 = ldnext(s + 7);
+        }
+#pragma unroll
+        for (int g = 0; g < NG; g++) {
+            cplx *y = a.Y + (int64_t)(16 * g + jl) * a.ld + rb * 16 + kl;
+#pragma unroll
+            for (int r = 0; r < 4; r++) {
+                double2 o;
+                o.x = acc[3 * g][r] - acc[3 * g + 2][r];
+                o.y = acc[3 * g][r] + acc[3 * g + 1][r];
+                *reinterpret_cast<double2 *>(y + 4 * r) = o;
+            }
+        }
+        __builtin_amdgcn_sched_barrier(0);
+    };
+    // The first block is peeled, so that the loop is entered with the
+    // stores of a block in flight as on every later pass: the first steps of
+    // a block then wait only for their own fragments (loaded before those
+    // stores), not for the stores to complete.
+    if (rb < nrb) {
+        const cplx *pn = lanep(rb + stride);
+        block(rb, pn);
+        pc = pn;
+        for (rb += stride; rb < nrb; rb += stride) {
+            pn = lanep(rb + stride);
+            block(rb, pn);
+            pc = pn;
+        }
+    }
+}
+
+// out[v] = sum_k sqrt(v_k) Y1[k][rank'(v - e_k)] + sum_P f_P(v) Y2[P][rank''(v - e_P)]
+template <int M>
+__global__ __launch_bounds__(PT) void k_pair_out(int N, int64_t nc, const int32_t *__restrict__ IF,
+                                                 const int32_t *__restrict__ Bg, int64_t nu1, int64_t nu2,
+                                                 int64_t ld2, const cplx *__restrict__ Y1,
+                                                 const cplx *__restrict__ Y2,
+                                                 cplx *__restrict__ out)
+{
+    extern __shared__ __align__(16) unsigned char lds[];
+    double *sq = (double *)lds;                      // [N + 3]
+    int32_t *Bt = (int32_t *)(sq + N + 3);          // [M][N + 1]
+    const int NB = N + 1;
+    for (int x = threadIdx.x; x < N + 3; x += PT) sq[x] = sqrt((double)x);
+    for (int x = threadIdx.x; x < M * NB; x += PT) Bt[x] = Bg[x];
+    __syncthreads();
+    const int64_t i = (int64_t)blockIdx.x * PT + threadIdx.x;
+    if (i >= nc) return;
+    int v[M];
+#pragma unroll
+    for (int k = 0; k < M; k++) v[k] = IF[i * M + k];
+    // S_d(x) = sum_{j=1..x} B[j][P_j - d] (P_j: particles below orbital j),
+    // modulo 2^32 (every rank is below 2^31)
+    uint32_t S0[M], S1[M], S2[M];
+    {
+        int pj = 0;
+        uint32_t s0 = 0, s1 = 0, s2 = 0;
+        S0[0] = S1[0] = S2[0] = 0;
+#pragma unroll
+        for (int j = 1; j < M; j++) {
+            pj += v[j - 1];
+            const int b = j * NB + pj;
+            s0 += (uint32_t)Bt[b];
+            s1 += (uint32_t)Bt[pj >= 1 ? b - 1 : b];
+            s2 += (uint32_t)Bt[pj >= 2 ? b - 2 : b];
+            S0[j] = s0;
+            S1[j] = s1;
+            S2[j] = s2;
+        }
+    }
+    cplx acc = cmake(0, 0);
+    if (nu1 > 0) {
+        const uint32_t top1 = (uint32_t)(nu1 - 1) - S1[M - 1];
+#pragma unroll
+        for (int k = 0; k < M; k++) {
+            if (v[k] < 1) continue;
+            const uint32_t u = top1 - (S0[k] - S1[k]);
+            const cplx y = ldcp(Y1 + (int64_t)k * nu1 + u);
+            acc.re = fma(sq[v[k]], y.re, acc.re);
+            acc.im = fma(sq[v[k]], y.im, acc.im);
+        }
+    }
+    if (nu2 > 0) {
+        const uint32_t top2 = (uint32_t)(nu2 - 1) - S2[M - 1];
+#pragma unroll
+        for (int c2 = 0; c2 < M; c2++)
+#pragma unroll
+            for (int c1 = 0; c1 <= c2; c1++) {
+                const bool occ = c1 < c2 ? (v[c1] >= 1 && v[c2] >= 1) : v[c1] >= 2;
+                if (!occ) continue;
+                const uint32_t u = top2 - (S0[c1] - S1[c1]) - (S1[c2] - S2[c2]);
+                const double f = c1 < c2 ? sq[v[c1]] * sq[v[c2]] : sq[v[c1]] * sq[v[c1] - 1];
+                const cplx y = ldcp(Y2 + (int64_t)(c2 * (c2 + 1) / 2 + c1) * ld2 + u);
+                acc.re = fma(f, y.re, acc.re);
+                acc.im = fma(f, y.im, acc.im);
+            }
+    }
+    double2 o;
+    o.x = acc.re;
+    o.y = acc.im;
+    *reinterpret_cast<double2 *>(out + i) = o;
+}
+
+template <int M, int U>
+mcpi_status launch_y1(const PairArgs &a, int ncu, hipStream_t st)
+{
+    const size_t sm = PairLds<M, U>::bytes(a.N);
+    MCPI_ARG(sm <= 160 * 1024, "pair-space tile exceeds LDS");
+    const int per_cu = std::max(1, (int)(160 * 1024 / sm));
+    const int64_t nb = std::max<int64_t>(1, std::min<int64_t>(a.nut, (int64_t)ncu * per_cu));
+    (void)hipFuncSetAttribute((const void *)k_pair_y1<M, U>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)sm);
+    hipLaunchKernelGGL((k_pair_y1<M, U>), dim3((unsigned)nb), dim3(PT), sm, st, a);
+    MCPI_HIP(hipGetLastError());
+    return MCPI_OK;
+}
+
+template <int M>
+mcpi_status launch_pair_m(const BasisDev &b, const HamCoefDev &c, const cplx *C, cplx *out, cplx *ws,
+                          hipStream_t st)
+{
+    int dev = 0, ncu = 0;
+    MCPI_HIP(hipGetDevice(&dev));
+    MCPI_HIP(hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev));
+    const int N = b.N;
+    const int64_t nu1 = N >= 1 ? nc_of(N - 1, M) : 0, nu2 = N >= 2 ? nc_of(N - 2, M) : 0;
+    const int64_t ld2 = pair_ld(nu2);
+    cplx *Y1 = ws, *Y2 = ws + (size_t)M * nu1;
+    if (nu1 > 0) {
+        PairArgs a{N, b.nc, nu1, (nu1 + 127) / 128, b.B, C, c.ph1, Y1, nu1};
+        MCPI_CHECK((launch_y1<M, 128>(a, ncu, st)));
+    }
+    if (nu2 > 0) {
+        // amplitudes X into the workspace after Y2, then Y2 = W X (rows of
+        // ld2 entries; Y2 has pair_stride(M) rows, X pair_x_rows(M))
+        constexpr int U = PairAmpLds<M>::U;
+        static_assert(16 * PairMulLds<M>::NG == pair_stride(M) && 4 * PairMulLds<M>::KS == pair_x_rows(M),
+                      "pair workspace rows");
+        cplx *X = Y2 + (size_t)pair_stride(M) * ld2;
+        PairArgs a{N, b.nc, nu2, (nu2 + U - 1) / U, b.B, C, c.pw, X, ld2};
+        const size_t sm = PairAmpLds<M>::bytes(N);
+        int per_cu = 0;
+        MCPI_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, (const void *)k_pair_amp<M>, PT, sm));
+        int64_t nb = std::max<int64_t>(1, std::min<int64_t>(a.nut, (int64_t)ncu * std::max(per_cu, 1)));
+        hipLaunchKernelGGL(k_pair_amp<M>, dim3((unsigned)nb), dim3(PT), sm, st, a);
+        MCPI_HIP(hipGetLastError());
+        a.Y = Y2;
+        const size_t smw = PairMulLds<M>::bytes;
+        MCPI_ARG(smw <= 160 * 1024, "pair matrix exceeds LDS");
+        (void)hipFuncSetAttribute((const void *)k_pair_mul<M>, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                  (int)smw);
+        const int nt = 64 * PAIR_MUL_WAVES;
+        MCPI_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, (const void *)k_pair_mul<M>, nt, smw));
+        const int64_t nrb = (nu2 + 15) / 16;
+        nb = std::max<int64_t>(1, std::min<int64_t>((nrb + PAIR_MUL_WAVES - 1) / PAIR_MUL_WAVES,
+                                                    (int64_t)ncu * std::max(per_cu, 1)));
+        hipLaunchKernelGGL(k_pair_mul<M>, dim3((unsigned)nb), dim3(nt), smw, st, a, (const cplx *)X);
+        MCPI_HIP(hipGetLastError());
+    }
+    const size_t sm = sizeof(double) * (N + 3) + sizeof(int32_t) * M * (N + 1);
+    hipLaunchKernelGGL(k_pair_out<M>, dim3((unsigned)((b.nc + PT - 1) / PT)), dim3(PT), sm, st, N, b.nc,
+                       (const int32_t *)b.IF, (const int32_t *)b.B, nu1, nu2, ld2, (const cplx *)Y1,
+                       (const cplx *)Y2, out);
+    MCPI_HIP(hipGetLastError());
+    return MCPI_OK;
+}
+
+}  // namespace
+
+// complex entries of the pair-space workspace of a basis: Y1 (M rows of
+// nu1), then Y2 (pair_stride(M) rows) and the pair amplitudes X
+// (pair_x_rows(M) rows) with rows of pair_ld(nu2) entries; the caller
+// zeroes it once (the pad rows and columns of X are read, never written)
+size_t pair_workspace_entries(int N, int M)
+{
+    const int64_t nu1 = N >= 1 ? nc_of(N - 1, M) : 0, nu2 = N >= 2 ? nc_of(N - 2, M) : 0;
+    return (size_t)M * nu1 + (size_t)(pair_stride(M) + pair_x_rows(M)) * pair_ld(nu2);
+}
+
+bool pair_applicable(const BasisDev &b, const HamCoefDev &c)
+{
+    return c.pw && c.ph1 && b.M >= 3 && b.M <= MCPI_PAIR_MAXM && !b.perm && b.IF && b.nc <= INT32_MAX &&
+           b.N <= MCPI_MAXN;
+}
+
+mcpi_status launch_pair(const BasisDev &b, const HamCoefDev &c, const cplx *C, cplx *out, cplx *ws,
+                        hipStream_t st)
+{
+    MCPI_ARG(pair_applicable(b, c), "the pair-space H.v does not apply to this basis");
+    switch (b.M) {
+    case 3: return launch_pair_m<3>(b, c, C, out, ws, st);
+    case 4: return launch_pair_m<4>(b, c, C, out, ws, st);
+    case 5: return launch_pair_m<5>(b, c, C, out, ws, st);
+    case 6: return launch_pair_m<6>(b, c, C, out, ws, st);
+    case 7: return launch_pair_m<7>(b, c, C, out, ws, st);
+    case 8: return launch_pair_m<8>(b, c, C, out, ws, st);
+    case 9: return launch_pair_m<9>(b, c, C, out, ws, st);
+    case 10: return launch_pair_m<10>(b, c, C, out, ws, st);
+    case 11: return launch_pair_m<11>(b, c, C, out, ws, st);
+    case 12: return launch_pair_m<12>(b, c, C, out, ws, st);
+    default: return MCPI_ERR_ARG;
+    }
+}
+
+}  // namespace mcpi
