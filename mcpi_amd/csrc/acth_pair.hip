@@ -54,6 +54,17 @@ __device__ inline cplx ldcp(const cplx *p)
     return cmake(t.x, t.y);
 }
 
+// 16-byte load at byte offset off of the buffer [p, p + bytes) (bytes <
+// 2^31); an offset at or past the end reads zero without a memory access
+__device__ inline cplx ldbuf(const cplx *p, int bytes, uint32_t off)
+{
+    typedef unsigned int v4u __attribute__((ext_vector_type(4)));
+    const __amdgpu_buffer_rsrc_t r = __builtin_amdgcn_make_buffer_rsrc((void *)p, (short)0, bytes, 0x00020000);
+    const v4u w = __builtin_amdgcn_raw_buffer_load_b128(r, off, 0, 0);
+    return cmake(__builtin_bit_cast(double, (unsigned long long)w.x | ((unsigned long long)w.y << 32)),
+                 __builtin_bit_cast(double, (unsigned long long)w.z | ((unsigned long long)w.w << 32)));
+}
+
 template <int M, int U>
 struct PairLds {
     static constexpr int RS = M + 1;   // LDS row stride of the tile (odd)
@@ -471,32 +482,60 @@ __global__ __launch_bounds__(PT) void k_pair_out(int N, int64_t nc, const int32_
             S2[j] = s2;
         }
     }
+    // Every term is read without a branch through a buffer load: an empty
+    // orbital (pair) gets an offset past its column, which the buffer's
+    // range check turns into a zero without a memory access, so the loads
+    // of a batch issue back to back and no lane touches an extra line.
+    constexpr uint32_t OUT = 0x80000000u;   // beyond every column (< 2 GiB)
     cplx acc = cmake(0, 0);
     if (nu1 > 0) {
         const uint32_t top1 = (uint32_t)(nu1 - 1) - S1[M - 1];
+        const int bytes1 = (int)(nu1 * (int64_t)sizeof(cplx));
+        cplx y[M];
 #pragma unroll
         for (int k = 0; k < M; k++) {
-            if (v[k] < 1) continue;
-            const uint32_t u = top1 - (S0[k] - S1[k]);
-            const cplx y = ldcp(Y1 + (int64_t)k * nu1 + u);
-            acc.re = fma(sq[v[k]], y.re, acc.re);
-            acc.im = fma(sq[v[k]], y.im, acc.im);
+            const uint32_t off = v[k] >= 1 ? (top1 - (S0[k] - S1[k])) * (uint32_t)sizeof(cplx) : OUT;
+            y[k] = ldbuf(Y1 + (int64_t)k * nu1, bytes1, off);
+        }
+#pragma unroll
+        for (int k = 0; k < M; k++) {
+            const double f = sq[v[k]];
+            acc.re = fma(f, y[k].re, acc.re);
+            acc.im = fma(f, y[k].im, acc.im);
         }
     }
     if (nu2 > 0) {
         const uint32_t top2 = (uint32_t)(nu2 - 1) - S2[M - 1];
+        const int bytes2 = (int)(nu2 * (int64_t)sizeof(cplx));
+        constexpr int NPp = pair_count(M), BT = 13;
 #pragma unroll
-        for (int c2 = 0; c2 < M; c2++)
+        for (int b0 = 0; b0 < NPp; b0 += BT) {
+            cplx y[BT];
+            double f[BT];
 #pragma unroll
-            for (int c1 = 0; c1 <= c2; c1++) {
+            for (int j = 0; j < BT; j++) {
+                const int P = b0 + j;
+                if (P >= NPp) break;
+                const int c2 = pair_hi(P), c1 = P - c2 * (c2 + 1) / 2;
                 const bool occ = c1 < c2 ? (v[c1] >= 1 && v[c2] >= 1) : v[c1] >= 2;
-                if (!occ) continue;
-                const uint32_t u = top2 - (S0[c1] - S1[c1]) - (S1[c2] - S2[c2]);
-                const double f = c1 < c2 ? sq[v[c1]] * sq[v[c2]] : sq[v[c1]] * sq[v[c1] - 1];
-                const cplx y = ldcp(Y2 + (int64_t)(c2 * (c2 + 1) / 2 + c1) * ld2 + u);
-                acc.re = fma(f, y.re, acc.re);
-                acc.im = fma(f, y.im, acc.im);
+                const uint32_t off =
+                    occ ? (top2 - (S0[c1] - S1[c1]) - (S1[c2] - S2[c2])) * (uint32_t)sizeof(cplx) : OUT;
+                f[j] = c1 < c2 ? sq[v[c1]] * sq[v[c2]] : sq[v[c1]] * sq[v[c1] >= 1 ? v[c1] - 1 : 0];
+                y[j] = ldbuf(Y2 + (int64_t)P * ld2, bytes2, off);
             }
+#pragma unroll
+            for (int j = 0; j < BT; j++) {
+                if (b0 + j >= NPp) break;
+                acc.re = fma(f[j], y[j].re, acc.re);
+                acc.im = fma(f[j], y[j].im, acc.im);
+            }
+            // an empty statement that takes the sums and fences memory, so
+            // that this batch is summed before the next batch's loads issue
+            // (all 78 loads up front take every register: one wavefront per
+            // SIMD; with the next batch in flight during the sums, two
+            // wavefronts per SIMD and no gain)
+            asm volatile("" : "+v"(acc.re), "+v"(acc.im) : : "memory");
+        }
     }
     double2 o;
     o.x = acc.re;
@@ -581,8 +620,10 @@ size_t pair_workspace_entries(int N, int M)
 
 bool pair_applicable(const BasisDev &b, const HamCoefDev &c)
 {
+    // (the output pass reads Y1 and Y2 columns as buffers of < 2 GiB:
+    // NC(N-1, M) >= NC(N-2, M) entries of 16 bytes)
     return c.pw && c.ph1 && b.M >= 3 && b.M <= MCPI_PAIR_MAXM && !b.perm && b.IF && b.nc <= INT32_MAX &&
-           b.N <= MCPI_MAXN;
+           b.N <= MCPI_MAXN && (b.N < 1 || nc_of(b.N - 1, b.M) < ((int64_t)1 << 27));
 }
 
 mcpi_status launch_pair(const BasisDev &b, const HamCoefDev &c, const cplx *C, cplx *out, cplx *ws,

@@ -263,6 +263,13 @@ __host__ __device__ inline HamCoefDev shift_coef(HamCoefDev c, ptrdiff_t d)
 // pair-space tables of schedule 6: M(M+1)/2 pairs, padded to 16
 __host__ __device__ constexpr int pair_count(int M) { return M * (M + 1) / 2; }
 __host__ __device__ constexpr int pair_stride(int M) { return (pair_count(M) + 15) / 16 * 16; }
+// the larger orbital c2 of pair P = c2 (c2 + 1) / 2 + c1 (c1 <= c2)
+__host__ __device__ constexpr int pair_hi(int P)
+{
+    int c2 = 0;
+    while ((c2 + 1) * (c2 + 2) / 2 <= P) c2++;
+    return c2;
+}
 // rows of the pair-amplitude matrix X of the pair-space H.v: the pairs,
 // rounded up to four (the rows past the pairs stay zero)
 __host__ __device__ constexpr int pair_x_rows(int M) { return (pair_count(M) + 3) / 4 * 4; }
