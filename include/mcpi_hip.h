@@ -287,7 +287,7 @@ mcpi_status mcpi_hip_ham_set_skip_zero(mcpi_ham h, int on);
  * multiplied by the Hamiltonian's pair matrix and one-body matrix, and each
  * row gathers its M(M+1)/2 + M products back (any storage mode, reads no
  * map; 3 <= M <= 12, bases in the reference order; a workspace of about
- * 2 M(M+1)/2 NC(N-2, M) + M NC(N-1, M) complex entries per stream that runs
+ * 2 M(M+1)/2 NC(N-2, M) complex entries per stream that runs
  * the schedule, allocated on that stream's first use outside stream capture,
  * is kept with the basis; falls back to 2 otherwise or when device memory
  * is short) */

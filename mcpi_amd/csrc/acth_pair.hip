@@ -46,6 +46,8 @@ struct PairArgs {
     const cplx *W;        // pair matrix pw[Q * PWS + P], or h1 as ph1[l * 16 + k]
     cplx *Y;              // Y[col * ld + u]
     int64_t ld;           // row stride of Y (and of the pair amplitudes X), >= nu
+    const cplx *H1;       // pair product: h1 (ph1[l * 16 + k]) folded into W with weight h1s
+    double h1s;           // 1 / (N - 1), or 0
 };
 
 __device__ inline cplx ldcp(const cplx *p)
@@ -319,6 +321,27 @@ typedef double mcpi_v4d __attribute__((ext_vector_type(4)));
 
 constexpr int PAIR_MUL_WAVES = 8;
 
+// The one-body operator in pair form: on states of N >= 2 particles
+//   a*_k a_l = (1 / (N - 1)) sum_s a*_k a*_s a_s a_l
+// (the number operator is N - 1 on a_l |v>), so h1 adds
+// h1[k][l] / (N - 1) to W[{k, s}][{s, l}] for every s.  This is the sum of
+// h1[k][l] over the orderings (k, s) of the pair P and (s, l) of the pair Q
+// that share s.
+__device__ inline cplx h1_pair(const cplx *h1, int P, int Q)
+{
+    const int p2 = pair_hi(P), p1 = P - p2 * (p2 + 1) / 2;
+    const int q2 = pair_hi(Q), q1 = Q - q2 * (q2 + 1) / 2;
+    cplx z = cmake(0, 0);
+    for (int o = 0; o < (p1 == p2 ? 1 : 2); o++) {
+        const int k = o ? p1 : p2, s = o ? p2 : p1;
+        for (int o2 = 0; o2 < (q1 == q2 ? 1 : 2); o2++) {
+            const int s2 = o2 ? q2 : q1, l = o2 ? q1 : q2;
+            if (s2 == s) z = cadd(z, ldcp(h1 + l * 16 + k));
+        }
+    }
+    return z;
+}
+
 template <int M>
 struct PairMulLds {
     static constexpr int NP = pair_count(M), KS = (NP + 3) / 4, NG = (NP + 15) / 16;
@@ -339,7 +362,12 @@ __global__ __launch_bounds__(64 * PAIR_MUL_WAVES) void k_pair_mul(PairArgs a, co
         double2 v = {0, 0};
         double vb = 0;
         if (P < NP && Q < NP) {
-            const cplx w = ldcp(a.W + (int64_t)Q * PWS + P);
+            cplx w = ldcp(a.W + (int64_t)Q * PWS + P);
+            if (a.h1s != 0.0) {
+                const cplx h = h1_pair(a.H1, P, Q);
+                w.re = fma(a.h1s, h.re, w.re);
+                w.im = fma(a.h1s, h.im, w.im);
+            }
             v.x = w.re;
             v.y = w.im - w.re;
             vb = w.re + w.im;
@@ -566,9 +594,12 @@ mcpi_status launch_pair_m(const BasisDev &b, const HamCoefDev &c, const cplx *C,
     const int N = b.N;
     const int64_t nu1 = N >= 1 ? nc_of(N - 1, M) : 0, nu2 = N >= 2 ? nc_of(N - 2, M) : 0;
     const int64_t ld2 = pair_ld(nu2);
-    cplx *Y1 = ws, *Y2 = ws + (size_t)M * nu1;
-    if (nu1 > 0) {
-        PairArgs a{N, b.nc, nu1, (nu1 + 127) / 128, b.B, C, c.ph1, Y1, nu1};
+    // with N >= 2 the one-body part rides in the pair product (h1 folded
+    // into W); the pass over N - 1 particles runs only for N = 1
+    const int64_t nu1p = N == 1 ? nu1 : 0;
+    cplx *Y1 = ws, *Y2 = ws + (size_t)M * nu1p;
+    if (nu1p > 0) {
+        PairArgs a{N, b.nc, nu1, (nu1 + 127) / 128, b.B, C, c.ph1, Y1, nu1, nullptr, 0.0};
         MCPI_CHECK((launch_y1<M, 128>(a, ncu, st)));
     }
     if (nu2 > 0) {
@@ -578,7 +609,7 @@ mcpi_status launch_pair_m(const BasisDev &b, const HamCoefDev &c, const cplx *C,
         static_assert(16 * PairMulLds<M>::NG == pair_stride(M) && 4 * PairMulLds<M>::KS == pair_x_rows(M),
                       "pair workspace rows");
         cplx *X = Y2 + (size_t)pair_stride(M) * ld2;
-        PairArgs a{N, b.nc, nu2, (nu2 + U - 1) / U, b.B, C, c.pw, X, ld2};
+        PairArgs a{N, b.nc, nu2, (nu2 + U - 1) / U, b.B, C, c.pw, X, ld2, c.ph1, 1.0 / (N - 1)};
         const size_t sm = PairAmpLds<M>::bytes(N);
         int per_cu = 0;
         MCPI_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, (const void *)k_pair_amp<M>, PT, sm));
@@ -600,7 +631,7 @@ mcpi_status launch_pair_m(const BasisDev &b, const HamCoefDev &c, const cplx *C,
     }
     const size_t sm = sizeof(double) * (N + 3) + sizeof(int32_t) * M * (N + 1);
     hipLaunchKernelGGL(k_pair_out<M>, dim3((unsigned)((b.nc + PT - 1) / PT)), dim3(PT), sm, st, N, b.nc,
-                       (const int32_t *)b.IF, (const int32_t *)b.B, nu1, nu2, ld2, (const cplx *)Y1,
+                       (const int32_t *)b.IF, (const int32_t *)b.B, nu1p, nu2, ld2, (const cplx *)Y1,
                        (const cplx *)Y2, out);
     MCPI_HIP(hipGetLastError());
     return MCPI_OK;
@@ -609,12 +640,13 @@ mcpi_status launch_pair_m(const BasisDev &b, const HamCoefDev &c, const cplx *C,
 }  // namespace
 
 // complex entries of the pair-space workspace of a basis: Y1 (M rows of
-// nu1), then Y2 (pair_stride(M) rows) and the pair amplitudes X
-// (pair_x_rows(M) rows) with rows of pair_ld(nu2) entries; the caller
-// zeroes it once (the pad rows and columns of X are read, never written)
+// NC(N-1, M), for N = 1 only), then Y2 (pair_stride(M) rows) and the pair
+// amplitudes X (pair_x_rows(M) rows) with rows of pair_ld(NC(N-2, M))
+// entries; the caller zeroes it once (the pad rows and columns of X are
+// read, never written)
 size_t pair_workspace_entries(int N, int M)
 {
-    const int64_t nu1 = N >= 1 ? nc_of(N - 1, M) : 0, nu2 = N >= 2 ? nc_of(N - 2, M) : 0;
+    const int64_t nu1 = N == 1 ? nc_of(N - 1, M) : 0, nu2 = N >= 2 ? nc_of(N - 2, M) : 0;
     return (size_t)M * nu1 + (size_t)(pair_stride(M) + pair_x_rows(M)) * pair_ld(nu2);
 }
 
