@@ -282,15 +282,23 @@ mcpi_status mcpi_hip_ham_set_skip_zero(mcpi_ham h, int on);
  * 5 = every term, with the double jumps of each block of 256 rows
  * compacted into full wavefronts before their targets are read from the
  * wave-packed tables (full maps; falls back to 2 otherwise),
- * 6 = every term through the pair space: the amplitudes <u|a_q a_l|C> and
- * <u|a_l|C> of the bases with N-2 and N-1 particles are gathered and
- * multiplied by the Hamiltonian's pair matrix and one-body matrix, and each
- * row gathers its M(M+1)/2 + M products back (any storage mode, reads no
+ * 6 = every term through the pair space: the pair amplitudes <u|a_q a_l|C>
+ * over the basis with N-2 particles are gathered and multiplied by the
+ * Hamiltonian's pair matrix (its one-body part folded in as
+ * a*_k a_l = sum_s a*_k a*_s a_s a_l / (N-1)), and each row gathers its
+ * M(M+1)/2 products back (any storage mode, reads no
  * map; 3 <= M <= 12, bases in the reference order; a workspace of about
  * 2 M(M+1)/2 NC(N-2, M) complex entries per stream that runs
  * the schedule, allocated on that stream's first use outside stream capture,
  * is kept with the basis; falls back to 2 otherwise or when device memory
- * is short) */
+ * is short), 7 = 6 in the factorized pair form, for Hamiltonians whose
+ * one-body part is diagonal and whose pair matrix is block diagonal with
+ * blocks of rank one, W[P][Q] = a[P] b[Q] (contact interactions that conserve
+ * a pair charge, such as Lieb-Liniger's): per block r, Z_r = sum_Q b[Q] a_Q C
+ * is gathered over the states of N-2 particles and each row gathers its
+ * a[P] Z_r back, with no matrix product (same workspace; falls back to 2
+ * otherwise).  The autotuner takes 6 and 7 among its candidates only with
+ * zero-skipping on, since they read no map */
 mcpi_status mcpi_hip_ham_set_schedule(mcpi_ham h, int schedule);
 /* schedule in use (after autotuning; 0 before the first H.v) */
 mcpi_status mcpi_hip_ham_get_schedule(mcpi_ham h, int *schedule);

@@ -186,11 +186,91 @@ mcpi_status build_coefficients(int M, const cplx *Ho, const cplx *Hint, HamCoefD
                     }
             }
     }
+    // The factorized pair form (schedule 7): h1 diagonal, and the pair
+    // matrix block diagonal over the connected components of its non-zero
+    // pattern with every block of rank one, W[P][Q] = za[P] zb[Q] (a contact
+    // interaction that conserves a pair charge: Lieb-Liniger's Hint = g
+    // delta(k + s, q + l) gives W = g/2 m m^T on each total momentum, m = 1
+    // or 2).  Then sum_{P,Q} W a*_P a_Q = sum_r A_r^+ B_r with
+    // B_r = sum_{Q in r} zb[Q] a_Q: zblk[P] is the block of P (-1: no term),
+    // zord lists the pairs block by block, zoff[r] .. zoff[r+1] its range.
+    std::vector<cplx> za, zb;
+    std::vector<int32_t> zblk, zord, zoff;
+    if (pair) {
+        const int NP = pair_count(M);
+        auto W = [&](int P, int Q) { return pw[(size_t)Q * PWS + P]; };
+        bool ok = true;
+        for (int k = 0; k < M && ok; k++)
+            for (int l = 0; l < M; l++)
+                if (l != k && nzc(ph1[(size_t)l * 16 + k])) ok = false;
+        std::vector<int> par(NP);
+        for (int x = 0; x < NP; x++) par[x] = x;
+        auto root = [&](int x) {
+            while (par[x] != x) x = par[x] = par[par[x]];
+            return x;
+        };
+        std::vector<uint8_t> used(NP, 0);
+        for (int P = 0; P < NP && ok; P++)
+            for (int Q = 0; Q < NP; Q++)
+                if (nzc(W(P, Q))) {
+                    used[P] = used[Q] = 1;
+                    par[root(P)] = root(Q);
+                }
+        zblk.assign(NP, -1);
+        std::vector<int> rid(NP, -1);
+        int R = 0;
+        for (int P = 0; P < NP && ok; P++)
+            if (used[P]) {
+                const int r = root(P);
+                if (rid[r] < 0) rid[r] = R++;
+                zblk[P] = rid[r];
+            }
+        za.assign(NP, cmake(0, 0));
+        zb.assign(NP, cmake(0, 0));
+        for (int r = 0; r < R && ok; r++) {
+            std::vector<int> mem;
+            for (int P = 0; P < NP; P++)
+                if (zblk[P] == r) mem.push_back(P);
+            int P0 = mem[0], Q0 = mem[0];
+            double big = -1;
+            for (int P : mem)
+                for (int Q : mem) {
+                    const double a = std::hypot(W(P, Q).re, W(P, Q).im);
+                    if (a > big) big = a, P0 = P, Q0 = Q;
+                }
+            const cplx w0 = W(P0, Q0);
+            const double n0 = w0.re * w0.re + w0.im * w0.im;
+            for (int P : mem) za[P] = W(P, Q0);
+            for (int Q : mem) {   // zb[Q] = W(P0, Q) / w0
+                const cplx w = W(P0, Q);
+                zb[Q] = cmake((w.re * w0.re + w.im * w0.im) / n0, (w.im * w0.re - w.re * w0.im) / n0);
+            }
+            for (int P : mem)
+                for (int Q : mem) {
+                    const cplx w = W(P, Q), f = cfma(za[P], zb[Q], cmake(0, 0));
+                    if (std::hypot(w.re - f.re, w.im - f.im) > 1e-13 * big) ok = false;
+                }
+        }
+        if (ok && R > 0) {
+            zoff.assign(R + 1, 0);
+            for (int P = 0; P < NP; P++)
+                if (zblk[P] >= 0) zoff[zblk[P] + 1]++;
+            for (int r = 0; r < R; r++) zoff[r + 1] += zoff[r];
+            zord.assign(std::max(zoff[R], 1), 0);
+            std::vector<int32_t> fill(zoff.begin(), zoff.end() - 1);
+            for (int P = 0; P < NP; P++)
+                if (zblk[P] >= 0) zord[fill[zblk[P]]++] = P;
+        } else {
+            za.clear();
+            zb.clear();
+            zblk.clear();
+        }
+    }
     // one device block: complex tables first (16-byte aligned), then integers
     size_t nC = dk.size() + d1.size() + d2.size() + A0.size() + A.size() + ot.size() + tt.size() + nz_c.size() +
-                otc.size() + ttc.size() + pw.size() + ph1.size();
+                otc.size() + ttc.size() + pw.size() + ph1.size() + za.size() + zb.size();
     size_t nB = nz1.size() + nzo.size() + nzt.size() + 4 * (m1j.size() + ot_off.size() + tt_off.size() +
-                                                           nz_e.size()) + 64;
+                                                           nz_e.size() + zblk.size() + zord.size() + zoff.size()) + 64;
     size_t bytes = (nC * sizeof(cplx) + nB + 64 + 255) & ~(size_t)255;
     char *dev = nullptr;
     MCPI_HIP(hipMalloc(&dev, bytes));
@@ -209,10 +289,15 @@ mcpi_status build_coefficients(int M, const cplx *Ho, const cplx *Hint, HamCoefD
     size_t o_ttc = put(ttc.data(), ttc.size() * sizeof(cplx));
     const size_t o_pw = pw.empty() ? 0 : put(pw.data(), pw.size() * sizeof(cplx));
     const size_t o_ph1 = ph1.empty() ? 0 : put(ph1.data(), ph1.size() * sizeof(cplx));
+    const size_t o_za = za.empty() ? 0 : put(za.data(), za.size() * sizeof(cplx));
+    const size_t o_zb = zb.empty() ? 0 : put(zb.data(), zb.size() * sizeof(cplx));
     size_t o_m1 = put(m1j.data(), m1j.size() * 4);
     size_t o_oo = put(ot_off.data(), ot_off.size() * 4);
     size_t o_to = put(tt_off.data(), tt_off.size() * 4);
     size_t o_ne = put(nz_e.data(), nz_e.size() * 4);
+    const size_t o_zblk = zblk.empty() ? 0 : put(zblk.data(), zblk.size() * 4);
+    const size_t o_zord = zblk.empty() ? 0 : put(zord.data(), zord.size() * 4);
+    const size_t o_zoff = zblk.empty() ? 0 : put(zoff.data(), zoff.size() * 4);
     size_t o_n1 = put(nz1.data(), nz1.size());
     size_t o_no = put(nzo.data(), nzo.size());
     size_t o_nt = put(nzt.data(), nzt.size());
@@ -232,6 +317,13 @@ mcpi_status build_coefficients(int M, const cplx *Ho, const cplx *Hint, HamCoefD
     c->ttc = (cplx *)(dev + o_ttc);
     c->pw = pair ? (cplx *)(dev + o_pw) : nullptr;
     c->ph1 = pair ? (cplx *)(dev + o_ph1) : nullptr;
+    const bool fac = !zblk.empty();
+    c->pza = fac ? (cplx *)(dev + o_za) : nullptr;
+    c->pzb = fac ? (cplx *)(dev + o_zb) : nullptr;
+    c->pzblk = fac ? (int32_t *)(dev + o_zblk) : nullptr;
+    c->pzord = fac ? (int32_t *)(dev + o_zord) : nullptr;
+    c->pzoff = fac ? (int32_t *)(dev + o_zoff) : nullptr;
+    c->pzR = fac ? (int)zoff.size() - 1 : 0;
     c->nz_1j = (uint8_t *)(dev + o_n1);
     c->nz_ot = (uint8_t *)(dev + o_no);
     c->nz_tt = (uint8_t *)(dev + o_nt);
@@ -596,12 +688,13 @@ static mcpi_status launch_sched(mcpi_ham h, int sched, const cplx *C, cplx *out,
         bd.ppoffTT = pt->offTT;
         return launch_apply_raw(bd, h->basis->maps, h->c, 1, 2, C, out, st);
     }
-    if (sched == 6) {
-        if (pair_schedule_ok(h)) {
+    if (sched == 6 || sched == 7) {
+        const bool ok = sched == 7 ? pairz_applicable(b, h->c) : pair_schedule_ok(h);
+        if (ok) {
             hipStreamCaptureStatus cs = hipStreamCaptureStatusNone;
             MCPI_HIP(hipStreamIsCapturing(st, &cs));
             cplx *ws = pair_workspace(h->basis, st, cs == hipStreamCaptureStatusNone);
-            if (ws) return launch_pair(b, h->c, C, out, ws, st);
+            if (ws) return sched == 7 ? launch_pairz(b, h->c, C, out, ws, st) : launch_pair(b, h->c, C, out, ws, st);
         }
         sched = 2;
     }
@@ -633,14 +726,19 @@ static mcpi_status autotune(mcpi_ham h, const cplx *C, cplx *out, hipStream_t st
         h->tuned[h->skip_zero ? 1 : 0] = 4;
         return MCPI_OK;
     }
-    int cands[6], n = 0;
+    int cands[8], n = 0;
     cands[n++] = 1;
     if (has_fast_kernel(b.N, b.M)) {
         cands[n++] = 2;
         if (maps == MCPI_MAPS_FULL) cands[n++] = 3;
         if (maps == MCPI_MAPS_FULL && b.packOT && b.packTT) cands[n++] = 5;
         if (pattern_schedule_ok(h) && basis_pattern(h->basis, h->c, true, st)) cands[n++] = 4;
-        if (pair_schedule_ok(h) && pair_workspace(h->basis, st, true)) cands[n++] = 6;
+        // the pair-space schedules read no map: candidates only when the
+        // caller has not asked for every term of every map (skip_zero off)
+        if (h->skip_zero && pair_schedule_ok(h) && pair_workspace(h->basis, st, true)) {
+            cands[n++] = 6;
+            if (pairz_applicable(b, h->c)) cands[n++] = 7;
+        }
     }
     int best = cands[n - 1];
     if (n > 1) {

@@ -48,6 +48,9 @@ struct PairArgs {
     int64_t ld;           // row stride of Y (and of the pair amplitudes X), >= nu
     const cplx *H1;       // pair product: h1 (ph1[l * 16 + k]) folded into W with weight h1s
     double h1s;           // 1 / (N - 1), or 0
+    const int32_t *zord;  // factorized form: pairs block by block (W = zb of the pairs)
+    const int32_t *zoff;  // block r at zord[zoff[r] .. zoff[r+1])
+    int zR;               // blocks
 };
 
 __device__ inline cplx ldcp(const cplx *p)
@@ -213,7 +216,10 @@ struct PairAmpLds {
     }
 };
 
-template <int M>
+// FAC: instead of X, Z[r][u] = sum_{Q in block r} zb[Q] X[Q][u] for the
+// R blocks of the factorized pair form (W = zb; a wavefront per block and
+// 64 rows, the block's pairs four at a time); Z has R rows of ld entries.
+template <int M, bool FAC>
 __global__ __launch_bounds__(PT) void k_pair_amp(PairArgs a)
 {
     constexpr int NP = pair_count(M), U = PairAmpLds<M>::U;
@@ -267,6 +273,48 @@ __global__ __launch_bounds__(PT) void k_pair_amp(PairArgs a)
             }
         }
         __syncthreads();
+        if (FAC) {
+            // the pair (q <= l) of index col and its amplitude's target and factor
+            auto amp = [&](int col, int ul, int64_t base, int64_t &tg, double &f) {
+                const uint8_t *P = pp + ul * (M + 1);
+                const int l = pair_hi(col), q = col - l * (l + 1) / 2;
+                const int uq = P[q + 1] - P[q], ulo = P[l + 1] - P[l];
+                tg = base - dq[ul * M + q] - dl[ul * M + l];
+                f = sq[uq + 1] * sq[ulo + 1 + (q == l)];
+            };
+            for (int e = t; e < a.zR * U; e += PT) {
+                const int r = e / U, ul = e % U;          // r: one per wavefront
+                const int64_t base = tt[ul];
+                if (base < 0) continue;
+                const int j0 = a.zoff[r], j1 = a.zoff[r + 1];
+                cplx acc = cmake(0, 0);
+                for (int j = j0; j < j1; j += 4) {
+                    int64_t tg[4];
+                    double f[4];
+                    cplx w[4], v[4];
+#pragma unroll
+                    for (int k = 0; k < 4; k++) {
+                        tg[k] = -1;
+                        f[k] = 0;
+                        w[k] = cmake(0, 0);
+                        if (j + k < j1) {
+                            const int col = a.zord[j + k];
+                            amp(col, ul, base, tg[k], f[k]);
+                            w[k] = ldcp(a.W + col);
+                        }
+                    }
+#pragma unroll
+                    for (int k = 0; k < 4; k++) v[k] = tg[k] >= 0 ? ldcp(a.C + tg[k]) : cmake(0, 0);
+#pragma unroll
+                    for (int k = 0; k < 4; k++) acc = cfma(cscale(w[k], f[k]), v[k], acc);
+                }
+                double2 o;
+                o.x = acc.re;
+                o.y = acc.im;
+                *reinterpret_cast<double2 *>(a.Y + (int64_t)r * a.ld + u0 + ul) = o;
+            }
+            continue;
+        }
         constexpr int NE = U * NP, GBP = 5;
         for (int e0 = t; e0 < NE; e0 += GBP * PT) {
             int64_t tg[GBP];
@@ -471,13 +519,16 @@ __global__ __launch_bounds__(64 * PAIR_MUL_WAVES) void k_pair_mul(PairArgs a, co
     }
 }
 
-// out[v] = sum_k sqrt(v_k) Y1[k][rank'(v - e_k)] + sum_P f_P(v) Y2[P][rank''(v - e_P)]
-template <int M>
+// out[v] = sum_k sqrt(v_k) Y1[k][rank'(v - e_k)] + sum_P f_P(v) Y2[P][rank''(v - e_P)];
+// FAC (factorized pair form, Y2 = Z with one row per block):
+// out[v] = (sum_k h1[k][k] v_k) C[v] + sum_P f_P(v) za[P] Z[zblk[P]][rank''(v - e_P)]
+template <int M, bool FAC>
 __global__ __launch_bounds__(PT) void k_pair_out(int N, int64_t nc, const int32_t *__restrict__ IF,
                                                  const int32_t *__restrict__ Bg, int64_t nu1, int64_t nu2,
                                                  int64_t ld2, const cplx *__restrict__ Y1,
-                                                 const cplx *__restrict__ Y2,
-                                                 cplx *__restrict__ out)
+                                                 const cplx *__restrict__ Y2, const cplx *__restrict__ za,
+                                                 const int32_t *__restrict__ zblk, const cplx *__restrict__ h1,
+                                                 const cplx *__restrict__ C, cplx *__restrict__ out)
 {
     extern __shared__ __align__(16) unsigned char lds[];
     double *sq = (double *)lds;                      // [N + 3]
@@ -516,6 +567,16 @@ __global__ __launch_bounds__(PT) void k_pair_out(int N, int64_t nc, const int32_
     // of a batch issue back to back and no lane touches an extra line.
     constexpr uint32_t OUT = 0x80000000u;   // beyond every column (< 2 GiB)
     cplx acc = cmake(0, 0);
+    if (FAC) {   // the diagonal one-body part
+        cplx d = cmake(0, 0);
+#pragma unroll
+        for (int k = 0; k < M; k++) {
+            const cplx h = ldcp(h1 + k * 17);
+            d.re = fma(h.re, (double)v[k], d.re);
+            d.im = fma(h.im, (double)v[k], d.im);
+        }
+        acc = cfma(d, ldcp(C + i), acc);
+    }
     if (nu1 > 0) {
         const uint32_t top1 = (uint32_t)(nu1 - 1) - S1[M - 1];
         const int bytes1 = (int)(nu1 * (int64_t)sizeof(cplx));
@@ -538,24 +599,30 @@ __global__ __launch_bounds__(PT) void k_pair_out(int N, int64_t nc, const int32_
         constexpr int NPp = pair_count(M), BT = 13;
 #pragma unroll
         for (int b0 = 0; b0 < NPp; b0 += BT) {
-            cplx y[BT];
+            cplx y[BT], cf[BT];
             double f[BT];
 #pragma unroll
             for (int j = 0; j < BT; j++) {
                 const int P = b0 + j;
                 if (P >= NPp) break;
                 const int c2 = pair_hi(P), c1 = P - c2 * (c2 + 1) / 2;
-                const bool occ = c1 < c2 ? (v[c1] >= 1 && v[c2] >= 1) : v[c1] >= 2;
+                const int blk = FAC ? zblk[P] : P;
+                const bool occ = (c1 < c2 ? (v[c1] >= 1 && v[c2] >= 1) : v[c1] >= 2) && blk >= 0;
                 const uint32_t off =
                     occ ? (top2 - (S0[c1] - S1[c1]) - (S1[c2] - S2[c2])) * (uint32_t)sizeof(cplx) : OUT;
                 f[j] = c1 < c2 ? sq[v[c1]] * sq[v[c2]] : sq[v[c1]] * sq[v[c1] >= 1 ? v[c1] - 1 : 0];
-                y[j] = ldbuf(Y2 + (int64_t)P * ld2, bytes2, off);
+                if (FAC) cf[j] = cscale(ldcp(za + P), f[j]);
+                y[j] = ldbuf(Y2 + (int64_t)(blk >= 0 ? blk : 0) * ld2, bytes2, off);
             }
 #pragma unroll
             for (int j = 0; j < BT; j++) {
                 if (b0 + j >= NPp) break;
-                acc.re = fma(f[j], y[j].re, acc.re);
-                acc.im = fma(f[j], y[j].im, acc.im);
+                if (FAC) {
+                    acc = cfma(cf[j], y[j], acc);
+                } else {
+                    acc.re = fma(f[j], y[j].re, acc.re);
+                    acc.im = fma(f[j], y[j].im, acc.im);
+                }
             }
             // an empty statement that takes the sums and fences memory, so
             // that this batch is summed before the next batch's loads issue
@@ -599,7 +666,7 @@ mcpi_status launch_pair_m(const BasisDev &b, const HamCoefDev &c, const cplx *C,
     const int64_t nu1p = N == 1 ? nu1 : 0;
     cplx *Y1 = ws, *Y2 = ws + (size_t)M * nu1p;
     if (nu1p > 0) {
-        PairArgs a{N, b.nc, nu1, (nu1 + 127) / 128, b.B, C, c.ph1, Y1, nu1, nullptr, 0.0};
+        PairArgs a{N, b.nc, nu1, (nu1 + 127) / 128, b.B, C, c.ph1, Y1, nu1, nullptr, 0.0, nullptr, nullptr, 0};
         MCPI_CHECK((launch_y1<M, 128>(a, ncu, st)));
     }
     if (nu2 > 0) {
@@ -609,12 +676,13 @@ mcpi_status launch_pair_m(const BasisDev &b, const HamCoefDev &c, const cplx *C,
         static_assert(16 * PairMulLds<M>::NG == pair_stride(M) && 4 * PairMulLds<M>::KS == pair_x_rows(M),
                       "pair workspace rows");
         cplx *X = Y2 + (size_t)pair_stride(M) * ld2;
-        PairArgs a{N, b.nc, nu2, (nu2 + U - 1) / U, b.B, C, c.pw, X, ld2, c.ph1, 1.0 / (N - 1)};
+        PairArgs a{N, b.nc, nu2, (nu2 + U - 1) / U, b.B, C, c.pw, X, ld2, c.ph1, 1.0 / (N - 1),
+                   nullptr, nullptr, 0};
         const size_t sm = PairAmpLds<M>::bytes(N);
         int per_cu = 0;
-        MCPI_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, (const void *)k_pair_amp<M>, PT, sm));
+        MCPI_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, (const void *)k_pair_amp<M, false>, PT, sm));
         int64_t nb = std::max<int64_t>(1, std::min<int64_t>(a.nut, (int64_t)ncu * std::max(per_cu, 1)));
-        hipLaunchKernelGGL(k_pair_amp<M>, dim3((unsigned)nb), dim3(PT), sm, st, a);
+        hipLaunchKernelGGL((k_pair_amp<M, false>), dim3((unsigned)nb), dim3(PT), sm, st, a);
         MCPI_HIP(hipGetLastError());
         a.Y = Y2;
         const size_t smw = PairMulLds<M>::bytes;
@@ -630,9 +698,39 @@ mcpi_status launch_pair_m(const BasisDev &b, const HamCoefDev &c, const cplx *C,
         MCPI_HIP(hipGetLastError());
     }
     const size_t sm = sizeof(double) * (N + 3) + sizeof(int32_t) * M * (N + 1);
-    hipLaunchKernelGGL(k_pair_out<M>, dim3((unsigned)((b.nc + PT - 1) / PT)), dim3(PT), sm, st, N, b.nc,
-                       (const int32_t *)b.IF, (const int32_t *)b.B, nu1p, nu2, ld2, (const cplx *)Y1,
-                       (const cplx *)Y2, out);
+    hipLaunchKernelGGL((k_pair_out<M, false>), dim3((unsigned)((b.nc + PT - 1) / PT)), dim3(PT), sm, st, N,
+                       b.nc, (const int32_t *)b.IF, (const int32_t *)b.B, nu1p, nu2, ld2, (const cplx *)Y1,
+                       (const cplx *)Y2, (const cplx *)nullptr, (const int32_t *)nullptr,
+                       (const cplx *)nullptr, (const cplx *)nullptr, out);
+    MCPI_HIP(hipGetLastError());
+    return MCPI_OK;
+}
+
+// the factorized pair form (schedule 7): Z (pzR rows of ld2) at the
+// start of the workspace, then the output pass
+template <int M>
+mcpi_status launch_pairz_m(const BasisDev &b, const HamCoefDev &c, const cplx *C, cplx *out, cplx *ws,
+                           hipStream_t st)
+{
+    int dev = 0, ncu = 0;
+    MCPI_HIP(hipGetDevice(&dev));
+    MCPI_HIP(hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev));
+    const int N = b.N;
+    const int64_t nu2 = nc_of(N - 2, M), ld2 = pair_ld(nu2);
+    cplx *Z = ws;
+    constexpr int U = PairAmpLds<M>::U;
+    PairArgs a{N, b.nc, nu2, (nu2 + U - 1) / U, b.B, C, c.pzb, Z, ld2, nullptr, 0.0, c.pzord, c.pzoff, c.pzR};
+    const size_t sm = PairAmpLds<M>::bytes(N);
+    int per_cu = 0;
+    MCPI_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, (const void *)k_pair_amp<M, true>, PT, sm));
+    const int64_t nb = std::max<int64_t>(1, std::min<int64_t>(a.nut, (int64_t)ncu * std::max(per_cu, 1)));
+    hipLaunchKernelGGL((k_pair_amp<M, true>), dim3((unsigned)nb), dim3(PT), sm, st, a);
+    MCPI_HIP(hipGetLastError());
+    const size_t so = sizeof(double) * (N + 3) + sizeof(int32_t) * M * (N + 1);
+    hipLaunchKernelGGL((k_pair_out<M, true>), dim3((unsigned)((b.nc + PT - 1) / PT)), dim3(PT), so, st, N, b.nc,
+                       (const int32_t *)b.IF, (const int32_t *)b.B, (int64_t)0, nu2, ld2, (const cplx *)nullptr,
+                       (const cplx *)Z, (const cplx *)c.pza, (const int32_t *)c.pzblk, (const cplx *)c.ph1, C,
+                       out);
     MCPI_HIP(hipGetLastError());
     return MCPI_OK;
 }
@@ -673,6 +771,30 @@ mcpi_status launch_pair(const BasisDev &b, const HamCoefDev &c, const cplx *C, c
     case 10: return launch_pair_m<10>(b, c, C, out, ws, st);
     case 11: return launch_pair_m<11>(b, c, C, out, ws, st);
     case 12: return launch_pair_m<12>(b, c, C, out, ws, st);
+    default: return MCPI_ERR_ARG;
+    }
+}
+
+bool pairz_applicable(const BasisDev &b, const HamCoefDev &c)
+{
+    return pair_applicable(b, c) && c.pzR > 0 && c.pza && b.N >= 2;
+}
+
+mcpi_status launch_pairz(const BasisDev &b, const HamCoefDev &c, const cplx *C, cplx *out, cplx *ws,
+                        hipStream_t st)
+{
+    MCPI_ARG(pairz_applicable(b, c), "the factorized pair-space H.v does not apply to this basis and Hamiltonian");
+    switch (b.M) {
+    case 3: return launch_pairz_m<3>(b, c, C, out, ws, st);
+    case 4: return launch_pairz_m<4>(b, c, C, out, ws, st);
+    case 5: return launch_pairz_m<5>(b, c, C, out, ws, st);
+    case 6: return launch_pairz_m<6>(b, c, C, out, ws, st);
+    case 7: return launch_pairz_m<7>(b, c, C, out, ws, st);
+    case 8: return launch_pairz_m<8>(b, c, C, out, ws, st);
+    case 9: return launch_pairz_m<9>(b, c, C, out, ws, st);
+    case 10: return launch_pairz_m<10>(b, c, C, out, ws, st);
+    case 11: return launch_pairz_m<11>(b, c, C, out, ws, st);
+    case 12: return launch_pairz_m<12>(b, c, C, out, ws, st);
     default: return MCPI_ERR_ARG;
     }
 }

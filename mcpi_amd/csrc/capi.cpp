@@ -193,7 +193,7 @@ mcpi_status mcpi_hip_ham_set_skip_zero(mcpi_ham h, int on)
 mcpi_status mcpi_hip_ham_set_schedule(mcpi_ham h, int schedule)
 {
     MCPI_ARG(h, "null handle");
-    MCPI_ARG(schedule >= 0 && schedule <= 6, "schedule must be 0 (auto), 1, 2, 3, 4, 5 or 6");
+    MCPI_ARG(schedule >= 0 && schedule <= 7, "schedule must be 0 (auto), 1, 2, 3, 4, 5, 6 or 7");
     MCPI_ARG(schedule == 0 || schedule == 4 || !h->basis->sectored(),
              "a sector-ordered basis runs schedule 4 (pattern tables) only");
     h->schedule = schedule;

@@ -226,6 +226,16 @@ struct HamCoefDev {
     // orbital contiguous, PWS = M(M+1)/2 rounded up to 16, zero padded)
     cplx *pw;
     cplx *ph1;
+    // factorized pair form (schedule 7; null / 0 unless h1 is diagonal and
+    // pw is block diagonal with rank-one blocks): pw[P][Q] = pza[P] pzb[Q]
+    // for P, Q in one block; pzblk[P] = block of P (-1: none); pzord lists
+    // the pairs of block r at pzoff[r] .. pzoff[r+1]; pzR blocks
+    cplx *pza;
+    cplx *pzb;
+    int32_t *pzblk;
+    int32_t *pzord;
+    int32_t *pzoff;
+    int pzR;
     int real;        // 1 if every coefficient above has a zero imaginary part
     int dense;       // 1 if at least half of the double-jump terms are non-zero:
                      // walking the lists would cost more than it skips
@@ -257,6 +267,13 @@ __host__ __device__ inline HamCoefDev shift_coef(HamCoefDev c, ptrdiff_t d)
     c.nz_c = sh(c.nz_c);
     if (c.pw) c.pw = sh(c.pw);
     if (c.ph1) c.ph1 = sh(c.ph1);
+    if (c.pza) {
+        c.pza = sh(c.pza);
+        c.pzb = sh(c.pzb);
+        c.pzblk = sh(c.pzblk);
+        c.pzord = sh(c.pzord);
+        c.pzoff = sh(c.pzoff);
+    }
     return c;
 }
 
@@ -344,6 +361,12 @@ bool pair_applicable(const BasisDev &b, const HamCoefDev &c);
 size_t pair_workspace_entries(int N, int M);
 mcpi_status launch_pair(const BasisDev &b, const HamCoefDev &c, const cplx *C, cplx *out, cplx *ws,
                         hipStream_t st);
+// schedule 7 (acth_pair.hip): the factorized pair form applies (as 6, and
+// the Hamiltonian's pair matrix factorizes, N >= 2); launch into the same
+// workspace
+bool pairz_applicable(const BasisDev &b, const HamCoefDev &c);
+mcpi_status launch_pairz(const BasisDev &b, const HamCoefDev &c, const cplx *C, cplx *out, cplx *ws,
+                         hipStream_t st);
 // build the requested maps of a basis whose IF table is in place
 mcpi_status basis_build_parts(mcpi_basis b, int want_map, int want_ot, int want_tt);
 

@@ -27,7 +27,9 @@ def schedules(M, maps):
     5 every term with each block's double jumps compacted into full
     wavefronts (full maps; otherwise it runs as 2); 6 every term through
     the pair space (two gathers around a dense product with the pair
-    matrix, any storage mode, M <= 12; otherwise it runs as 2)."""
+    matrix, any storage mode, M <= 12; otherwise it runs as 2); 7 the pair
+    space in the factorized form (diagonal one-body part and a pair matrix
+    with rank-one blocks, as Lieb-Liniger's; otherwise it runs as 2)."""
     out = [1]
     if 3 <= M <= 13:
         out.append(2)
@@ -37,6 +39,7 @@ def schedules(M, maps):
             out.append(4)
         out.append(5)
         out.append(6)
+        out.append(7)
     return out
 
 
@@ -117,6 +120,36 @@ def test_pair_schedule_two_streams(oracle_lib):
     finally:
         for s in streams:
             hip.hipStreamDestroy(s)
+
+
+@pytest.mark.parametrize("N,M", [(2, 3), (3, 5), (7, 8), (9, 12)])
+def test_factorized_pair_form(oracle_lib, N, M):
+    """Schedule 7 on Hamiltonians that factorize (a contact interaction
+    conserving momentum with a random real strength per total momentum K,
+    and a random real diagonal Ho) and on ones that do not (an off-diagonal
+    Ho, a dense Hint: the schedule falls back to 2), each against the
+    oracle."""
+    note(f"factorized pair form N={N} M={M}")
+    o = oracle.Basis(N, M)
+    C, _, _ = oracle.random_inputs(N, M, seed=N * 31 + M)
+    rng = np.random.default_rng(M)
+    gK = rng.standard_normal(2 * M) + 1j * rng.standard_normal(2 * M)
+    gK = gK + np.conj(gK)            # real: Hint Hermitian, W_K = g_K / 2 m m^T
+    k = np.arange(M)
+    K = k[:, None, None, None] + k[None, :, None, None]
+    Ksum = k[None, None, :, None] + k[None, None, None, :]
+    Hint = np.where(K == Ksum, gK[np.minimum(K, 2 * M - 1)], 0).astype(complex)
+    Hd = np.diag(rng.standard_normal(M)).astype(complex)
+    _, Ho_dense, Hint_dense = oracle.random_inputs(N, M, seed=5)
+    b = mcpi_amd.Basis(N, M, "single")
+    for label, Ho, Hi in (("factorizes", Hd, Hint), ("off-diagonal Ho", Ho_dense, Hint),
+                          ("dense Hint", Hd, Hint_dense)):
+        ref = o.acth(C, Ho, Hi)
+        for skip in (True, False):
+            H = mcpi_amd.Hamiltonian(b, Ho, Hi, skip_zero=skip, schedule=7)
+            assert_close(H(C), ref, 1e-12, f"{label} skip={skip}")
+        H = mcpi_amd.Hamiltonian(b, Ho, Hi)   # autotuned
+        assert_close(H(C), ref, 1e-12, f"{label} auto")
 
 
 @pytest.mark.parametrize("N,M,g", [(5, 11, 10.0), (8, 9, 0.3), (6, 12, 50.0)])
@@ -288,7 +321,7 @@ def test_n12_m12_generic_sampled_rows(oracle_lib):
     out = H(C)
     rows = np.concatenate([[0, b.nc - 1], rng.integers(0, b.nc, 1500)])
     assert_close(out[rows], oracle.acth_hash_rows(N, M, rows, C, Ho, Hint), 1e-12, "rows")
-    for sched in (1, 2, 3, 5, 6):
+    for sched in (1, 2, 3, 5, 6, 7):
         H2 = mcpi_amd.Hamiltonian(b, Ho, Hint, skip_zero=False, schedule=sched)
         assert_close(H2(C), out, 1e-12, f"schedule {sched}")
 
@@ -307,6 +340,6 @@ def test_n16_m12_single_map_sampled_rows(oracle_lib):
     rows = np.concatenate([[0, b.nc - 1], rng.integers(0, b.nc, 3000)])
     assert_close(out[rows], oracle.acth_hash_rows(N, M, rows, C, Ho, Hint), 1e-12, "rows")
     _hermitian_check(H, b.nc, 9)
-    for sched in (2, 4, 6):
+    for sched in (2, 4, 6, 7):
         H2 = mcpi_amd.Hamiltonian(b, Ho, Hint, skip_zero=True, schedule=sched)
         assert_close(H2(C), out, 1e-12, f"schedule {sched}")
