@@ -218,7 +218,7 @@ struct PairAmpLds {
 
 // FAC: instead of X, Z[r][u] = sum_{Q in block r} zb[Q] X[Q][u] for the
 // R blocks of the factorized pair form (W = zb; a wavefront per block and
-// 64 rows, the block's pairs four at a time); Z has R rows of ld entries.
+// 64 rows); Z has R rows of ld entries.
 template <int M, bool FAC>
 __global__ __launch_bounds__(PT) void k_pair_amp(PairArgs a)
 {
@@ -288,12 +288,17 @@ __global__ __launch_bounds__(PT) void k_pair_amp(PairArgs a)
                 if (base < 0) continue;
                 const int j0 = a.zoff[r], j1 = a.zoff[r + 1];
                 cplx acc = cmake(0, 0);
-                for (int j = j0; j < j1; j += 4) {
-                    int64_t tg[4];
-                    double f[4];
-                    cplx w[4], v[4];
+                // the block's pairs two at a time: a small register count
+                // (many wavefronts per SIMD) hides the gathers better than
+                // more loads per thread (four: +4% at N=16/M=12, eight and
+                // two outputs per thread: +20%)
+                constexpr int GBK = 2;
+                for (int j = j0; j < j1; j += GBK) {
+                    int64_t tg[GBK];
+                    double f[GBK];
+                    cplx w[GBK], v[GBK];
 #pragma unroll
-                    for (int k = 0; k < 4; k++) {
+                    for (int k = 0; k < GBK; k++) {
                         tg[k] = -1;
                         f[k] = 0;
                         w[k] = cmake(0, 0);
@@ -304,9 +309,9 @@ __global__ __launch_bounds__(PT) void k_pair_amp(PairArgs a)
                         }
                     }
 #pragma unroll
-                    for (int k = 0; k < 4; k++) v[k] = tg[k] >= 0 ? ldcp(a.C + tg[k]) : cmake(0, 0);
+                    for (int k = 0; k < GBK; k++) v[k] = tg[k] >= 0 ? ldcp(a.C + tg[k]) : cmake(0, 0);
 #pragma unroll
-                    for (int k = 0; k < 4; k++) acc = cfma(cscale(w[k], f[k]), v[k], acc);
+                    for (int k = 0; k < GBK; k++) acc = cfma(cscale(w[k], f[k]), v[k], acc);
                 }
                 double2 o;
                 o.x = acc.re;
