@@ -464,7 +464,7 @@ def large_basis(reps: int = 20, lm: int = 200, quick: bool = False):
         H = mcpi_amd.Hamiltonian(b, Ho, Hint, **kw)
         H.apply(dC, dO)                      # schedule tuning
         ms = H.apply_timed(dC, dO, reps)
-        if H.schedule == 6:
+        if label == "every_term_pair_space":
             out[label] = {"ms_per_hv": round(ms, 4), "schedule": 6,
                           "pair_product_flops": pair_product_flops(N, M)}
         else:
@@ -628,6 +628,7 @@ def main() -> int:
                           ("packed_rows_per_lane", dict(skip_zero=False, schedule=2)),
                           ("ranked_doubles", dict(skip_zero=False, schedule=3)),
                           ("pair_space_every_term", dict(skip_zero=False, schedule=6)),
+                          ("pair_space_factorized", dict(skip_zero=True, schedule=7)),
                           ("skip_zero_autotuned", dict(skip_zero=True, schedule=0))):
             Hx = mcpi_amd.Hamiltonian(b, Ho, Hint, **kw)
             for _ in range(2):
@@ -635,7 +636,7 @@ def main() -> int:
             others[label] = {"ms_per_hv": round(Hx.apply_timed(dC, dO, reps), 4), "schedule": Hx.schedule}
             if Hx.schedule == 4:
                 others[label]["pattern_table_bytes"] = b.pattern_bytes()
-            if Hx.schedule == 6:
+            if label == "pair_space_every_term":
                 # no map is read: the pair product's FP64 flops (Gauss form,
                 # three real products per complex one) are its model
                 others[label]["pair_product_flops"] = pair_product_flops(N_, M_)
