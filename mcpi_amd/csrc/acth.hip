@@ -141,15 +141,16 @@ mcpi_status build_coefficients(int M, const cplx *Ho, const cplx *Hint, HamCoefD
     //   OT, TT        a*_k a*_k a_q a_l,  a*_k a*_s a_q a_l
     // so the pair matrix collects the two-body coefficients and h1 the
     // one-body ones.  Both are uploaded with the other tables.
-    const bool pair = M >= 3 && M <= MCPI_PAIR_MAXM;
+    const bool pair = M >= 3 && M <= MCPI_PAIRZ_MAXM;
     const int PWS = pair_stride(M);
-    std::vector<cplx> pw(pair ? (size_t)PWS * PWS : 0, cmake(0, 0)), ph1(pair ? 16 * 16 : 0, cmake(0, 0));
+    const int HS = pair_h1_stride(M);
+    std::vector<cplx> pw(pair ? (size_t)PWS * PWS : 0, cmake(0, 0)), ph1(pair ? (size_t)HS * HS : 0, cmake(0, 0));
     if (pair) {
         auto addw = [&](int c1, int c2, int a1, int a2, cplx z) {
             cplx &w = pw[(size_t)pair_of2(a1, a2) * PWS + pair_of2(c1, c2)];
             w = cadd(w, z);
         };
-        auto addh = [&](int k, int l, cplx z) { ph1[(size_t)l * 16 + k] = cadd(ph1[(size_t)l * 16 + k], z); };
+        auto addh = [&](int k, int l, cplx z) { ph1[(size_t)l * HS + k] = cadd(ph1[(size_t)l * HS + k], z); };
         for (int k = 0; k < M; k++) {
             addh(k, k, dk[k]);
             addw(k, k, k, k, d1[k]);
@@ -202,7 +203,7 @@ mcpi_status build_coefficients(int M, const cplx *Ho, const cplx *Hint, HamCoefD
         bool ok = true;
         for (int k = 0; k < M && ok; k++)
             for (int l = 0; l < M; l++)
-                if (l != k && nzc(ph1[(size_t)l * 16 + k])) ok = false;
+                if (l != k && nzc(ph1[(size_t)l * HS + k])) ok = false;
         std::vector<int> par(NP);
         for (int x = 0; x < NP; x++) par[x] = x;
         auto root = [&](int x) {
@@ -715,6 +716,8 @@ static mcpi_status launch_sched(mcpi_ham h, int sched, const cplx *C, cplx *out,
         }
         sched = 2;
     }
+    // every fallback lands on 2, which exists for 3 <= M <= 13 only
+    if (sched >= 2 && !has_fast_kernel(b.N, b.M)) sched = 1;
     return launch_apply_raw(b, h->basis->maps, h->c, h->skip_zero, sched, C, out, st);
 }
 
@@ -735,11 +738,9 @@ static mcpi_status autotune(mcpi_ham h, const cplx *C, cplx *out, hipStream_t st
         if (pattern_schedule_ok(h) && basis_pattern(h->basis, h->c, true, st)) cands[n++] = 4;
         // the pair-space schedules read no map: candidates only when the
         // caller has not asked for every term of every map (skip_zero off)
-        if (h->skip_zero && pair_schedule_ok(h) && pair_workspace(h->basis, st, true)) {
-            cands[n++] = 6;
-            if (pairz_applicable(b, h->c)) cands[n++] = 7;
-        }
+        if (h->skip_zero && pair_schedule_ok(h) && pair_workspace(h->basis, st, true)) cands[n++] = 6;
     }
+    if (h->skip_zero && pairz_applicable(b, h->c) && pair_workspace(h->basis, st, true)) cands[n++] = 7;
     int best = cands[n - 1];
     if (n > 1) {
         hipEvent_t e0, e1;

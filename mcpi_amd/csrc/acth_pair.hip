@@ -43,7 +43,7 @@ struct PairArgs {
     int64_t nut;          // tiles of U rows
     const int32_t *B;     // rank table B[j*(N+1) + p]
     const cplx *C;
-    const cplx *W;        // pair matrix pw[Q * PWS + P], or h1 as ph1[l * 16 + k]
+    const cplx *W;        // pair matrix pw[Q * PWS + P], or h1 as ph1[l * 16 + k] (M <= 12)
     cplx *Y;              // Y[col * ld + u]
     int64_t ld;           // row stride of Y (and of the pair amplitudes X), >= nu
     const cplx *H1;       // pair product: h1 (ph1[l * 16 + k]) folded into W with weight h1s
@@ -576,7 +576,7 @@ __global__ __launch_bounds__(PT) void k_pair_out(int N, int64_t nc, const int32_
         cplx d = cmake(0, 0);
 #pragma unroll
         for (int k = 0; k < M; k++) {
-            const cplx h = ldcp(h1 + k * 17);
+            const cplx h = ldcp(h1 + k * (pair_h1_stride(M) + 1));
             d.re = fma(h.re, (double)v[k], d.re);
             d.im = fma(h.im, (double)v[k], d.im);
         }
@@ -666,6 +666,7 @@ mcpi_status launch_pair_m(const BasisDev &b, const HamCoefDev &c, const cplx *C,
     const int N = b.N;
     const int64_t nu1 = N >= 1 ? nc_of(N - 1, M) : 0, nu2 = N >= 2 ? nc_of(N - 2, M) : 0;
     const int64_t ld2 = pair_ld(nu2);
+    static_assert(pair_h1_stride(M) == 16, "the pair product reads h1 with row stride 16");
     // with N >= 2 the one-body part rides in the pair product (h1 folded
     // into W); the pass over N - 1 particles runs only for N = 1
     const int64_t nu1p = N == 1 ? nu1 : 0;
@@ -750,6 +751,8 @@ mcpi_status launch_pairz_m(const BasisDev &b, const HamCoefDev &c, const cplx *C
 size_t pair_workspace_entries(int N, int M)
 {
     const int64_t nu1 = N == 1 ? nc_of(N - 1, M) : 0, nu2 = N >= 2 ? nc_of(N - 2, M) : 0;
+    if (M > MCPI_PAIR_MAXM)   // the factorized form only: Z, at most one row per pair
+        return (size_t)pair_stride(M) * pair_ld(nu2);
     return (size_t)M * nu1 + (size_t)(pair_stride(M) + pair_x_rows(M)) * pair_ld(nu2);
 }
 
@@ -782,7 +785,8 @@ mcpi_status launch_pair(const BasisDev &b, const HamCoefDev &c, const cplx *C, c
 
 bool pairz_applicable(const BasisDev &b, const HamCoefDev &c)
 {
-    return pair_applicable(b, c) && c.pzR > 0 && c.pza && b.N >= 2;
+    return c.pza && c.pzR > 0 && b.M >= 3 && b.M <= MCPI_PAIRZ_MAXM && !b.perm && b.IF && b.nc <= INT32_MAX &&
+           b.N >= 2 && b.N <= MCPI_MAXN && nc_of(b.N - 1, b.M) < ((int64_t)1 << 27);
 }
 
 mcpi_status launch_pairz(const BasisDev &b, const HamCoefDev &c, const cplx *C, cplx *out, cplx *ws,
@@ -800,6 +804,18 @@ mcpi_status launch_pairz(const BasisDev &b, const HamCoefDev &c, const cplx *C, 
     case 10: return launch_pairz_m<10>(b, c, C, out, ws, st);
     case 11: return launch_pairz_m<11>(b, c, C, out, ws, st);
     case 12: return launch_pairz_m<12>(b, c, C, out, ws, st);
+    case 13: return launch_pairz_m<13>(b, c, C, out, ws, st);
+    case 14: return launch_pairz_m<14>(b, c, C, out, ws, st);
+    case 15: return launch_pairz_m<15>(b, c, C, out, ws, st);
+    case 16: return launch_pairz_m<16>(b, c, C, out, ws, st);
+    case 17: return launch_pairz_m<17>(b, c, C, out, ws, st);
+    case 18: return launch_pairz_m<18>(b, c, C, out, ws, st);
+    case 19: return launch_pairz_m<19>(b, c, C, out, ws, st);
+    case 20: return launch_pairz_m<20>(b, c, C, out, ws, st);
+    case 21: return launch_pairz_m<21>(b, c, C, out, ws, st);
+    case 22: return launch_pairz_m<22>(b, c, C, out, ws, st);
+    case 23: return launch_pairz_m<23>(b, c, C, out, ws, st);
+    case 24: return launch_pairz_m<24>(b, c, C, out, ws, st);
     default: return MCPI_ERR_ARG;
     }
 }

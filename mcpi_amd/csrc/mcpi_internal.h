@@ -222,7 +222,7 @@ struct HamCoefDev {
     // null otherwise): H = sum_{k,l} h1[k][l] a*_k a_l
     //                    + sum_{P,Q} pw[P][Q] a*_{P1} a*_{P2} a_{Q1} a_{Q2}
     // over unordered orbital pairs P, Q (index p = hi(hi+1)/2 + lo), stored
-    // pw[Q * PWS + P] and ph1[l * 16 + k] (column of an annihilated pair or
+    // pw[Q * PWS + P] and ph1[l * HS + k], HS = pair_h1_stride(M) (column of an annihilated pair or
     // orbital contiguous, PWS = M(M+1)/2 rounded up to 16, zero padded)
     cplx *pw;
     cplx *ph1;
@@ -280,6 +280,9 @@ __host__ __device__ inline HamCoefDev shift_coef(HamCoefDev c, ptrdiff_t d)
 // pair-space tables of schedule 6: M(M+1)/2 pairs, padded to 16
 __host__ __device__ constexpr int pair_count(int M) { return M * (M + 1) / 2; }
 __host__ __device__ constexpr int pair_stride(int M) { return (pair_count(M) + 15) / 16 * 16; }
+// row stride of the one-body table ph1 (h1[k][l] at l * stride + k): 16 up to
+// 16 orbitals, 32 beyond
+__host__ __device__ constexpr int pair_h1_stride(int M) { return M <= 16 ? 16 : 32; }
 // the larger orbital c2 of pair P = c2 (c2 + 1) / 2 + c1 (c1 <= c2)
 __host__ __device__ constexpr int pair_hi(int P)
 {
@@ -297,6 +300,8 @@ __host__ __device__ inline int pair_of2(int a, int b)
     return a < b ? b * (b + 1) / 2 + a : a * (a + 1) / 2 + b;
 }
 #define MCPI_PAIR_MAXM 12
+// the factorized pair form (schedule 7) has no product in LDS: larger M
+#define MCPI_PAIRZ_MAXM 24
 
 struct mcpi_ham_s {
     mcpi_basis basis;

@@ -122,7 +122,7 @@ def test_pair_schedule_two_streams(oracle_lib):
             hip.hipStreamDestroy(s)
 
 
-@pytest.mark.parametrize("N,M", [(2, 3), (3, 5), (7, 8), (9, 12)])
+@pytest.mark.parametrize("N,M", [(2, 3), (3, 5), (7, 8), (9, 12), (6, 13), (4, 16), (3, 21), (2, 24)])
 def test_factorized_pair_form(oracle_lib, N, M):
     """Schedule 7 on Hamiltonians that factorize (a contact interaction
     conserving momentum with a random real strength per total momentum K,
@@ -150,6 +150,26 @@ def test_factorized_pair_form(oracle_lib, N, M):
             assert_close(H(C), ref, 1e-12, f"{label} skip={skip}")
         H = mcpi_amd.Hamiltonian(b, Ho, Hi)   # autotuned
         assert_close(H(C), ref, 1e-12, f"{label} auto")
+
+
+@pytest.mark.parametrize("maps", ["full", "single", "none"])
+def test_large_m_schedules(oracle_lib, maps):
+    """M = 15 (beyond the kernels specialised on M <= 13): every schedule
+    either applies (1; 7 for the Lieb-Liniger Hamiltonian) or falls back
+    to the generic kernel, and all agree with the oracle."""
+    N, M = 3, 15
+    note(f"large M schedules N={N} M={M} maps={maps}")
+    o = oracle.Basis(N, M)
+    C, Ho, Hint = oracle.random_inputs(N, M, seed=315)
+    b = mcpi_amd.Basis(N, M, maps)
+    for label, ho, hi in (("random", Ho, Hint), ("Lieb-Liniger", oracle.ll_ho(M), oracle.ll_hint(M, 3.0))):
+        ref = o.acth(C, ho, hi)
+        for sched in range(1, 8):
+            H = mcpi_amd.Hamiltonian(b, ho, hi, schedule=sched)
+            assert_close(H(C), ref, 1e-12, f"{label} schedule {sched}")
+        H = mcpi_amd.Hamiltonian(b, ho, hi)
+        assert_close(H(C), ref, 1e-12, f"{label} auto")
+        assert H.schedule in ((1, 7) if label == "Lieb-Liniger" else (1,))
 
 
 @pytest.mark.parametrize("N,M,g", [(5, 11, 10.0), (8, 9, 0.3), (6, 12, 50.0)])
