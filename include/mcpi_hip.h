@@ -296,9 +296,11 @@ mcpi_status mcpi_hip_ham_set_skip_zero(mcpi_ham h, int on);
  * blocks of rank one, W[P][Q] = a[P] b[Q] (contact interactions that conserve
  * a pair charge, such as Lieb-Liniger's): per block r, Z_r = sum_Q b[Q] a_Q C
  * is gathered over the states of N-2 particles and each row gathers its
- * a[P] Z_r back, with no matrix product (same workspace; falls back to 2
- * otherwise).  The autotuner takes 6 and 7 among its candidates only with
- * zero-skipping on, since they read no map */
+ * a[P] Z_r back, with no matrix product (3 <= M <= 24, N >= 2; same
+ * workspace; falls back to 2 otherwise).  The autotuner takes 6 and 7 among
+ * its candidates only with zero-skipping on, since they read no map.  Where
+ * the kernels specialised on M do not exist (M > 13), every fallback is the
+ * generic kernel 1 */
 mcpi_status mcpi_hip_ham_set_schedule(mcpi_ham h, int schedule);
 /* schedule in use (after autotuning; 0 before the first H.v) */
 mcpi_status mcpi_hip_ham_get_schedule(mcpi_ham h, int *schedule);
