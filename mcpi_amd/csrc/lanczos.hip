@@ -731,7 +731,8 @@ struct HvGroup {
     std::vector<int> members;
     bool batched = false;
     bool multi_ok = false;        // members share the zero pattern: multi-vector kernel allowed
-    int mode = 1;                 // 1 batched specialised kernel, 2 multi-vector kernel
+    int mode = 1;                 // 1 batched specialised kernel, 2 multi-vector kernel,
+                                  // 3 each member on its own with its tuned schedule
     int sched = 2, skip = 0;
     size_t cbytes = 0;            // coefficient block size = stride
     HamCoefDev c0{};              // member 0's tables inside `coefs`
@@ -742,6 +743,13 @@ struct HvGroup {
 static mcpi_status launch_group(const HvGroup &g, const mcpi_ham *hams, const std::vector<LzInst> &hin,
                                 hipStream_t s)
 {
+    if (g.batched && g.mode == 3) {
+        // each member's own H.v (its autotuned schedule, the factorized
+        // pair form where it applies); the members share the basis's pair
+        // workspace on this stream one after the other
+        for (int x : g.members) MCPI_CHECK(launch_apply(hams[x], hin[x].vin, hin[x].HC, s));
+        return MCPI_OK;
+    }
     if (g.batched) {
         BasisDev bd = g.basis->d;
         int sched = g.sched;
@@ -772,8 +780,9 @@ static mcpi_status launch_group(const HvGroup &g, const mcpi_ham *hams, const st
 // Time the candidate (kernel, schedule) pairs on the whole group and keep
 // the fastest: the batched specialised kernel with the double-jump targets
 // read (2) or ranked (3), and, when the members share their zero pattern,
-// the multi-vector kernel with the same two schedules.  fixed: a schedule
-// the caller set on the Hamiltonians (0: free).
+// the multi-vector kernel with the same two schedules; where the factorized
+// pair form applies, also the members one by one with their own tuned H.v.
+// fixed: a schedule the caller set on the Hamiltonians (0: free).
 // kernel choice for batched groups: 0 = timed, 1 = batched specialised
 // kernel only, 2 = multi-vector kernel where the group allows it
 static std::atomic<int> g_batched_kernel{0};
@@ -812,6 +821,11 @@ static mcpi_status tune_group(HvGroup &g, const mcpi_ham *hams, const std::vecto
         }
     }
     if (cand.empty()) cand.push_back({1, fixed ? fixed : 2});
+    // the members one by one, each with its own tuned H.v: pays one launch
+    // pair per member, and wins where the factorized pair form applies
+    if (!fixed && g_batched_kernel == 0 && !g.basis->sectored() &&
+        pairz_applicable(g.basis->d, hams[g.members[0]]->c))
+        cand.push_back({3, 0});
     if (cand.size() == 1) {
         g.mode = cand[0].first;
         g.sched = cand[0].second;
