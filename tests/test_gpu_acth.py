@@ -82,34 +82,39 @@ def test_apply_sparse_hint_all_schedules(oracle_lib):
                 assert_close(H(C), ref, 1e-12, f"{maps} {sched} {skip}")
 
 
-def test_pair_schedule_two_streams(oracle_lib):
-    """Schedule 6 keeps one workspace per stream: two Hamiltonians on one
-    basis, their H.v enqueued alternately on two non-blocking streams
-    without synchronization in between (so that they may overlap), each
-    match the oracle; then the same on the default stream."""
+@pytest.mark.parametrize("sched", [6, 7])
+def test_pair_schedule_two_streams(oracle_lib, sched):
+    """Schedules 6 and 7 keep one workspace per stream: two Hamiltonians on
+    one basis (random ones for 6, Lieb-Liniger with two couplings for 7),
+    their H.v enqueued alternately on two non-blocking streams without
+    synchronization in between (so that they may overlap), each match the
+    oracle; then the same on the default stream."""
     import ctypes as ct
     hip = ct.CDLL("libamdhip64.so")
     N, M = 9, 10
     o = oracle.Basis(N, M)
     C1, Ho1, Hint1 = oracle.random_inputs(N, M, seed=61)
     C2, Ho2, Hint2 = oracle.random_inputs(N, M, seed=62)
+    if sched == 7:
+        Ho1, Hint1 = oracle.ll_ho(M), oracle.ll_hint(M, 2.0)
+        Ho2, Hint2 = oracle.ll_ho(M), oracle.ll_hint(M, 30.0)
     ref1, ref2 = o.acth(C1, Ho1, Hint1), o.acth(C2, Ho2, Hint2)
     b = mcpi_amd.Basis(N, M, "single")
-    H1 = mcpi_amd.Hamiltonian(b, Ho1, Hint1, skip_zero=False, schedule=6)
-    H2 = mcpi_amd.Hamiltonian(b, Ho2, Hint2, skip_zero=False, schedule=6)
+    H1 = mcpi_amd.Hamiltonian(b, Ho1, Hint1, skip_zero=False, schedule=sched)
+    H2 = mcpi_amd.Hamiltonian(b, Ho2, Hint2, skip_zero=False, schedule=sched)
     d1, d2 = mcpi_amd.DeviceArray.from_numpy(C1), mcpi_amd.DeviceArray.from_numpy(C2)
     o1, o2 = mcpi_amd.DeviceArray(b.nc), mcpi_amd.DeviceArray(b.nc)
     streams = [ct.c_void_p(), ct.c_void_p()]
     for s in streams:
         assert hip.hipStreamCreateWithFlags(ct.byref(s), ct.c_uint(1)) == 0
     try:
-        note(f"pair schedule two streams N={N} M={M}")
+        note(f"pair schedule {sched} two streams N={N} M={M}")
         for _ in range(4):
             H1.apply(d1, o1, streams[0])
             H2.apply(d2, o2, streams[1])
         for s in streams:
             assert hip.hipStreamSynchronize(s) == 0
-        assert H1.schedule == 6 and H2.schedule == 6
+        assert H1.schedule == sched and H2.schedule == sched
         assert_close(o1.numpy(), ref1, 1e-12, "stream 0")
         assert_close(o2.numpy(), ref2, 1e-12, "stream 1")
         H1.apply(d2, o1)
