@@ -1,28 +1,33 @@
-// H.v through the pair space (schedule 6): every term of the reference's
-// operator (hamiltonianMatrix.h:1431-1970, the folded rules of acth.hip),
-// evaluated as two gathers around a dense product instead of one gather per
-// term.
+// H.v through the pair space (schedules 6 and 7): every term of the
+// reference's operator (hamiltonianMatrix.h:1431-1970, the folded rules of
+// acth.hip), evaluated as two gathers around a pair-space step instead of
+// one gather per term.
 //
 // In pair form (build_coefficients) the operator is
 //   H = sum_{k,l} h1[k][l] a*_k a_l + sum_{P,Q} W[P][Q] a*_{P1} a*_{P2} a_{Q1} a_{Q2}
-// over unordered orbital pairs.  Inserting the identity over the basis with
-// N-2 (N-1) particles after the creators,
-//   (H C)(v) = sum_P <v|a*_{P1} a*_{P2}|u> Y2_u(P) + sum_k <v|a*_k|u'> Y1_u'(k),
-//   Y2_u(P)  = sum_Q W[P][Q] X[Q][u],  X[Q][u] = <u|a_{Q1} a_{Q2}|C>,  u  = v - e_{P1} - e_{P2}
-//   Y1_u'(k) = sum_l h1[k][l] <u'|a_l|C>,                             u' = v - e_k
+// over unordered orbital pairs.  On N >= 2 particles the one-body part is a
+// pair term too (a*_k a_l = sum_s a*_k a*_s a_s a_l / (N - 1)).  Inserting
+// the identity over the basis with N-2 particles after the creators,
+//   (H C)(v) = sum_P <v|a*_{P1} a*_{P2}|u> Y2_u(P),  u = v - e_{P1} - e_{P2}
+//   Y2_u(P)  = sum_Q W[P][Q] X[Q][u],  X[Q][u] = <u|a_{Q1} a_{Q2}|C>
 // and <u|a_q a_l|C> = sqrt((u_q + 1)(u_l + 1 + [q = l])) C[rank(u + e_q + e_l)]
 // is the pair amplitude of the densities' Gram matrix (rho.hip).  At N = M =
-// 12 a row has up to 1,636 double-jump terms; here it costs 78 + 12 output
-// gathers, and the 352,716 rows of 10 particles cost 78 gathers and a
-// 78 x 78 complex matrix-vector product each, on the FP64 MFMA.
+// 12 a row has up to 1,636 double-jump terms; here it costs 78 output
+// gathers, and the 352,716 rows of 10 particles cost 78 gathers and a 78 x
+// 78 complex matrix-vector product each, on the FP64 MFMA (schedule 6).
+// When W is block diagonal with rank-one blocks, W = a b^T on each block r
+// (a contact interaction that conserves a pair charge), and h1 is diagonal,
+// Y2_u(P) = a[P] Z[r(P)][u] with Z[r][u] = sum_{Q in r} b[Q] X[Q][u]: one
+// sum per block and no product (schedule 7).
 //
-// k_pair_y1<M, U>: Y1 for tiles of U rows of the basis with N-1 particles
-// (amplitudes gathered into LDS, then h1 times them with h1 in LDS).
-// k_pair_amp<M>: X, gathered.  k_pair_mul<M>: Y2 = W X on the MFMA.
-// k_pair_out<M>: one thread per row v of the basis ranks the v - e_P and
-// v - e_k in the smaller bases from its prefix-count tables and sums the
-// factors times Y.  Y1, Y2 and X are stored column-major (Y[col * ld + u])
-// so that every pass reads and writes them coalesced.
+// k_pair_amp<M, false>: X, gathered.  k_pair_mul<M>: Y2 = W X on the MFMA.
+// k_pair_amp<M, true>: Z, gathered and summed per block.
+// k_pair_out<M, FAC>: one thread per row v of the basis ranks the v - e_P
+// in the smaller basis from its prefix-count tables and sums the factors
+// times Y2 (or a[P] Z).  k_pair_y1<M, U>: for N = 1 only, the one-body
+// part over the empty basis of N - 1 particles.  Y2, Z and X are stored
+// column-major (Y[col * ld + u]) so that every pass reads and writes them
+// coalesced.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
