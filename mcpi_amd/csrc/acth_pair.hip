@@ -607,7 +607,10 @@ __global__ __launch_bounds__(PT) void k_pair_out(int N, int64_t nc, const int32_
         const uint32_t top2 = (uint32_t)(nu2 - 1) - S2[M - 1];
         const int bytes2 = (int)(nu2 * (int64_t)sizeof(cplx));
         constexpr int NPp = pair_count(M), BT = 13;
-#pragma unroll
+        // full unroll: every index below is then a constant (a partial
+        // unroll, which plain `unroll` gives from M = 15 on, indexes v and
+        // the prefix sums at run time and puts them in scratch)
+#pragma clang loop unroll(full)
         for (int b0 = 0; b0 < NPp; b0 += BT) {
             cplx y[BT], cf[BT];
             double f[BT];
