@@ -23,6 +23,7 @@
 #include <vector>
 
 #include "mcpi_internal.h"
+#include "pair_factor.h"
 
 namespace mcpi {
 
@@ -187,84 +188,17 @@ mcpi_status build_coefficients(int M, const cplx *Ho, const cplx *Hint, HamCoefD
                     }
             }
     }
-    // The factorized pair form (schedule 7): h1 diagonal, and the pair
-    // matrix block diagonal over the connected components of its non-zero
-    // pattern with every block of rank one, W[P][Q] = za[P] zb[Q] (a contact
-    // interaction that conserves a pair charge: Lieb-Liniger's Hint = g
-    // delta(k + s, q + l) gives W = g/2 m m^T on each total momentum, m = 1
-    // or 2).  Then sum_{P,Q} W a*_P a_Q = sum_r A_r^+ B_r with
-    // B_r = sum_{Q in r} zb[Q] a_Q: zblk[P] is the block of P (-1: no term),
-    // zord lists the pairs block by block, zoff[r] .. zoff[r+1] its range.
+    // the factorized pair form of schedule 7 (pair_factor.h)
     std::vector<cplx> za, zb;
     std::vector<int32_t> zblk, zord, zoff;
     if (pair) {
-        const int NP = pair_count(M);
-        auto W = [&](int P, int Q) { return pw[(size_t)Q * PWS + P]; };
-        bool ok = true;
-        for (int k = 0; k < M && ok; k++)
-            for (int l = 0; l < M; l++)
-                if (l != k && nzc(ph1[(size_t)l * HS + k])) ok = false;
-        std::vector<int> par(NP);
-        for (int x = 0; x < NP; x++) par[x] = x;
-        auto root = [&](int x) {
-            while (par[x] != x) x = par[x] = par[par[x]];
-            return x;
-        };
-        std::vector<uint8_t> used(NP, 0);
-        for (int P = 0; P < NP && ok; P++)
-            for (int Q = 0; Q < NP; Q++)
-                if (nzc(W(P, Q))) {
-                    used[P] = used[Q] = 1;
-                    par[root(P)] = root(Q);
-                }
-        zblk.assign(NP, -1);
-        std::vector<int> rid(NP, -1);
-        int R = 0;
-        for (int P = 0; P < NP && ok; P++)
-            if (used[P]) {
-                const int r = root(P);
-                if (rid[r] < 0) rid[r] = R++;
-                zblk[P] = rid[r];
-            }
-        za.assign(NP, cmake(0, 0));
-        zb.assign(NP, cmake(0, 0));
-        for (int r = 0; r < R && ok; r++) {
-            std::vector<int> mem;
-            for (int P = 0; P < NP; P++)
-                if (zblk[P] == r) mem.push_back(P);
-            int P0 = mem[0], Q0 = mem[0];
-            double big = -1;
-            for (int P : mem)
-                for (int Q : mem) {
-                    const double a = std::hypot(W(P, Q).re, W(P, Q).im);
-                    if (a > big) big = a, P0 = P, Q0 = Q;
-                }
-            const cplx w0 = W(P0, Q0);
-            const double n0 = w0.re * w0.re + w0.im * w0.im;
-            for (int P : mem) za[P] = W(P, Q0);
-            for (int Q : mem) {   // zb[Q] = W(P0, Q) / w0
-                const cplx w = W(P0, Q);
-                zb[Q] = cmake((w.re * w0.re + w.im * w0.im) / n0, (w.im * w0.re - w.re * w0.im) / n0);
-            }
-            for (int P : mem)
-                for (int Q : mem) {
-                    const cplx w = W(P, Q), f = cfma(za[P], zb[Q], cmake(0, 0));
-                    if (std::hypot(w.re - f.re, w.im - f.im) > 1e-13 * big) ok = false;
-                }
-        }
-        if (ok && R > 0) {
-            zoff.assign(R + 1, 0);
-            for (int P = 0; P < NP; P++)
-                if (zblk[P] >= 0) zoff[zblk[P] + 1]++;
-            for (int r = 0; r < R; r++) zoff[r + 1] += zoff[r];
-            zord.assign(std::max(zoff[R], 1), 0);
-            std::vector<int32_t> fill(zoff.begin(), zoff.end() - 1);
-            for (int P = 0; P < NP; P++)
-                if (zblk[P] >= 0) zord[fill[zblk[P]]++] = P;
-        } else {
-            za.clear();
-            zb.clear();
-            zblk.clear();
+        PairFactor<cplx> pf;
+        if (pair_factorize(M, pair_count(M), PWS, HS, pw.data(), ph1.data(), pf)) {
+            za = std::move(pf.za);
+            zb = std::move(pf.zb);
+            zblk = std::move(pf.zblk);
+            zord = std::move(pf.zord);
+            zoff = std::move(pf.zoff);
         }
     }
     // one device block: complex tables first (16-byte aligned), then integers
