@@ -296,9 +296,12 @@ mcpi_status mcpi_hip_ham_set_skip_zero(mcpi_ham h, int on);
  * blocks of rank one, W[P][Q] = a[P] b[Q] (contact interactions that conserve
  * a pair charge, such as Lieb-Liniger's): per block r, Z_r = sum_Q b[Q] a_Q C
  * is gathered over the states of N-2 particles and each row gathers its
- * a[P] Z_r back, with no matrix product (3 <= M <= 24, N >= 2; same
- * workspace; falls back to 2 otherwise).  The autotuner takes 6 and 7 among
- * its candidates only with zero-skipping on, since they read no map.  Where
+ * a[P] Z_r back, with no matrix product (3 <= M <= 24, N >= 2; a
+ * workspace of R NC(N-2, M) complex entries per stream, R the number of
+ * blocks, kept likewise; a smaller workspace stays allocated beside a larger
+ * one until the basis is freed; falls back to 2 otherwise).  The autotuner
+ * takes 6 and 7 among its candidates only with zero-skipping on, since they
+ * read no map, and 7 in place of 6 where it applies.  Where
  * the kernels specialised on M do not exist (M > 13), every fallback is the
  * generic kernel 1 */
 mcpi_status mcpi_hip_ham_set_schedule(mcpi_ham h, int schedule);

@@ -405,9 +405,12 @@ class Hamiltonian:
         specialised ranking double jumps on the fly, 4 specialised reading
         the pattern tables of the zero pattern (zero-skipping), 5 every
         term with each block's double jumps compacted into full wavefronts
-        (full maps), 6 every term through the pair space (the amplitudes of
-        the bases with N-2 and N-1 particles times the pair and one-body
-        matrices, gathered back per row; 3 <= M <= 12)."""
+        (full maps), 6 every term through the pair space (the two-particle
+        annihilation amplitudes over the basis with N-2 particles times
+        the pair matrix, the one-body part folded into it, gathered back
+        per row; 3 <= M <= 12), 7 the same in the factorized pair form
+        (diagonal one-body part, pair matrix block diagonal with blocks of
+        rank one, as for Lieb-Liniger; 3 <= M <= 24, N >= 2)."""
         _check(lib().mcpi_hip_ham_set_schedule(self.handle, int(schedule)), "schedule")
 
     @property

@@ -565,23 +565,38 @@ static bool pattern_schedule_ok(mcpi_ham h)
     return h->skip_zero && !h->c.dense && h->basis->maps >= MCPI_MAPS_SINGLE && has_fast_kernel(b.N, b.M);
 }
 
-// the basis's pair-space workspace (schedule 6) for stream st, allocated
-// when absent and allowed (not under capture, memory free beyond the
-// reserve); nullptr when not available
-static cplx *pair_workspace(mcpi_basis b, hipStream_t st, bool may_alloc)
+// complex entries of the pair-space workspace schedule `sched` writes: the
+// factorized form (7) only its Z rows, one per block of the pair matrix
+static size_t pair_need(mcpi_ham h, int sched)
+{
+    const BasisDev &b = h->basis->d;
+    if (sched == 7) return (size_t)h->c.pzR * pair_ld(nc_of(b.N - 2, b.M));
+    return pair_workspace_entries(b.N, b.M);
+}
+
+// the basis's pair-space workspace for stream st with at least `entries`
+// complex entries, allocated when absent or too small, if allowed (not
+// under capture, memory free beyond the reserve); nullptr when not
+// available. A smaller one stays allocated beside a larger one until the
+// basis is freed: a graph the caller captured may still point at it.
+static cplx *pair_workspace(mcpi_basis b, hipStream_t st, bool may_alloc, size_t entries)
 {
     std::lock_guard<std::mutex> lock(b->pair_mu);
-    for (const auto &w : b->pair_ws)
-        if (w.st == st) return (cplx *)w.p;
+    const size_t need = sizeof(cplx) * std::max<size_t>(entries, 1);
+    for (const auto &w : b->pair_ws) {
+        if (w.st != st) continue;
+        size_t have = 0;
+        if (hipMemPtrGetInfo(w.p, &have) == hipSuccess && have >= need) return (cplx *)w.p;
+        (void)hipGetLastError();
+    }
     if (!may_alloc) return nullptr;
-    const size_t need = sizeof(cplx) * pair_workspace_entries(b->d.N, b->d.M);
     size_t free_b = 0, total_b = 0;
     if (hipMemGetInfo(&free_b, &total_b) != hipSuccess || free_b < need + ((size_t)4 << 30)) {
         (void)hipGetLastError();
         return nullptr;
     }
     void *p = nullptr;
-    if (hipMalloc(&p, std::max<size_t>(need, 16)) != hipSuccess) {
+    if (hipMalloc(&p, need) != hipSuccess) {
         (void)hipGetLastError();
         return nullptr;
     }
@@ -628,7 +643,7 @@ static mcpi_status launch_sched(mcpi_ham h, int sched, const cplx *C, cplx *out,
         if (ok) {
             hipStreamCaptureStatus cs = hipStreamCaptureStatusNone;
             MCPI_HIP(hipStreamIsCapturing(st, &cs));
-            cplx *ws = pair_workspace(h->basis, st, cs == hipStreamCaptureStatusNone);
+            cplx *ws = pair_workspace(h->basis, st, cs == hipStreamCaptureStatusNone, pair_need(h, sched));
             if (ws) return sched == 7 ? launch_pairz(b, h->c, C, out, ws, st) : launch_pair(b, h->c, C, out, ws, st);
         }
         sched = 2;
@@ -670,11 +685,17 @@ static mcpi_status autotune(mcpi_ham h, const cplx *C, cplx *out, hipStream_t st
         if (maps == MCPI_MAPS_FULL) cands[n++] = 3;
         if (maps == MCPI_MAPS_FULL && b.packOT && b.packTT) cands[n++] = 5;
         if (pattern_schedule_ok(h) && basis_pattern(h->basis, h->c, true, st)) cands[n++] = 4;
-        // the pair-space schedules read no map: candidates only when the
-        // caller has not asked for every term of every map (skip_zero off)
-        if (h->skip_zero && pair_schedule_ok(h) && pair_workspace(h->basis, st, true)) cands[n++] = 6;
     }
-    if (h->skip_zero && pairz_applicable(b, h->c) && pair_workspace(h->basis, st, true)) cands[n++] = 7;
+    // the pair-space schedules read no map: candidates only when the caller
+    // has not asked for every term of every map (skip_zero off); the
+    // factorized form, where it applies, does less work than the dense
+    // product in a smaller workspace, and stands in for it
+    if (h->skip_zero && pairz_applicable(b, h->c)) {
+        if (pair_workspace(h->basis, st, true, pair_need(h, 7))) cands[n++] = 7;
+    } else if (h->skip_zero && has_fast_kernel(b.N, b.M) && pair_schedule_ok(h) &&
+               pair_workspace(h->basis, st, true, pair_need(h, 6))) {
+        cands[n++] = 6;
+    }
     int best = cands[n - 1];
     if (n > 1) {
         hipEvent_t e0, e1;

@@ -759,8 +759,6 @@ mcpi_status launch_pairz_m(const BasisDev &b, const HamCoefDev &c, const cplx *C
 size_t pair_workspace_entries(int N, int M)
 {
     const int64_t nu1 = N == 1 ? nc_of(N - 1, M) : 0, nu2 = N >= 2 ? nc_of(N - 2, M) : 0;
-    if (M > MCPI_PAIR_MAXM)   // the factorized form only: Z, at most one row per pair
-        return (size_t)pair_stride(M) * pair_ld(nu2);
     return (size_t)M * nu1 + (size_t)(pair_stride(M) + pair_x_rows(M)) * pair_ld(nu2);
 }
 

@@ -167,9 +167,11 @@ struct mcpi_basis_s {
     size_t rho_ws_bytes = 0;
     std::mutex rho_mu;
     // pair-space H.v workspaces (schedule 6: the pair amplitudes and the Y
-    // tables of the bases with N-1 and N-2 particles), one per stream that
-    // ran the schedule (so that H.v calls on different streams never share
-    // one), allocated on first use and kept
+    // tables of the bases with N-1 and N-2 particles; schedule 7: the Z
+    // rows), per stream that ran a pair-space schedule (so that H.v calls
+    // on different streams never share one), allocated on first use, a
+    // larger one beside a smaller one when the schedule needs it, and kept
+    // until the basis is freed
     struct PairWs {
         hipStream_t st;
         void *p;
@@ -367,8 +369,8 @@ size_t pair_workspace_entries(int N, int M);
 mcpi_status launch_pair(const BasisDev &b, const HamCoefDev &c, const cplx *C, cplx *out, cplx *ws,
                         hipStream_t st);
 // schedule 7 (acth_pair.hip): the factorized pair form applies (as 6, and
-// the Hamiltonian's pair matrix factorizes, N >= 2); launch into the same
-// workspace
+// the Hamiltonian's pair matrix factorizes, N >= 2); launch into a
+// workspace of (blocks) x pair_ld(NC(N-2, M)) complex entries
 bool pairz_applicable(const BasisDev &b, const HamCoefDev &c);
 mcpi_status launch_pairz(const BasisDev &b, const HamCoefDev &c, const cplx *C, cplx *out, cplx *ws,
                          hipStream_t st);

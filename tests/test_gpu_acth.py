@@ -127,6 +127,28 @@ def test_pair_schedule_two_streams(oracle_lib, sched):
             hip.hipStreamDestroy(s)
 
 
+def test_pair_workspace_sizes(oracle_lib):
+    """Schedule 7 allocates only its Z rows; schedule 6 on the same stream
+    then needs a larger workspace (allocated beside the first), and 7 after
+    it reuses either: every H.v matches the oracle.  The autotuner, with
+    zero-skipping on a Hamiltonian that factorizes, never picks 6."""
+    N, M = 9, 10
+    o = oracle.Basis(N, M)
+    C, _, _ = oracle.random_inputs(N, M, seed=63)
+    Ho, Hint = oracle.ll_ho(M), oracle.ll_hint(M, 5.0)
+    ref = o.acth(C, Ho, Hint)
+    b = mcpi_amd.Basis(N, M, "none")
+    H = mcpi_amd.Hamiltonian(b, Ho, Hint, skip_zero=False, schedule=7)
+    note(f"pair workspace sizes N={N} M={M}")
+    for sched in (7, 6, 7, 6):
+        H.set_schedule(sched)
+        assert_close(H(C), ref, 1e-12, f"schedule {sched}")
+        assert H.schedule == sched
+    Ht = mcpi_amd.Hamiltonian(mcpi_amd.Basis(N, M, "none"), Ho, Hint, skip_zero=True)
+    assert_close(Ht(C), ref, 1e-12, "autotuned")
+    assert Ht.schedule != 6
+
+
 @pytest.mark.parametrize("N,M", [(2, 3), (3, 5), (7, 8), (9, 12), (6, 13), (4, 16), (3, 21), (2, 24)])
 def test_factorized_pair_form(oracle_lib, N, M):
     """Schedule 7 on Hamiltonians that factorize (a contact interaction
