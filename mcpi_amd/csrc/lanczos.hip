@@ -318,21 +318,26 @@ __global__ __launch_bounds__(LZ_T) void lz_sub_alpha(const LanczosState *sts, co
 
 // 6b: partial o_t = <v_t, HC> for t = 0..i+1.  Block (s, g, instance) owns
 // row segment s (LZ_R rows per thread, kept in registers) and basis vectors
-// t in [g*LZ_TG, (g+1)*LZ_TG); waves reduce with shuffles and park their sums
-// in LDS, one barrier per block.
+// t in [g*LZ_DTG, (g+1)*LZ_DTG); waves reduce with shuffles and park their
+// sums in LDS, one barrier per block.  16 vectors per block rather than 32
+// give twice the blocks at a given step: over steps 4..121 on MI355X, 5.5
+// against 4.5 TB/s at 646,646 rows, 6.4 against 6.3 at 13.0 M rows or with
+// 24 instances of 646,646 rows side by side.
 #define LZ_R 8
-#define LZ_TG 32
+#define LZ_DTG 16
 #define LZ_SEG ((int64_t)LZ_T * LZ_R)
-__global__ __launch_bounds__(LZ_T) void lz_gemv_dots(const LanczosState *sts, const LzInst *in, int lm)
+// vector groups of the split Gram-Schmidt update (small bases)
+#define LZ_TG 32
+__global__ __launch_bounds__(LZ_T) void lz_gemv_dots(const LanczosState *sts, const LzInst *in)
 {
-    __shared__ double red[LZ_TG][LZ_T / 64][2];
+    __shared__ double red[LZ_DTG][LZ_T / 64][2];
     const LanczosState *st = sts + blockIdx.z;
     const LzInst &I = in[blockIdx.z];
     if ((int)blockIdx.x >= I.nbg || st->broke) return;
     const int nt = st->it + 2;
-    const int t0 = blockIdx.y * LZ_TG;
+    const int t0 = blockIdx.y * LZ_DTG;
     if (t0 >= nt) return;
-    const int t1 = min(nt, t0 + LZ_TG);
+    const int t1 = min(nt, t0 + LZ_DTG);
     const int64_t n = I.n;
     const int64_t base = (int64_t)blockIdx.x * LZ_SEG + threadIdx.x;
     cplx h[LZ_R];
@@ -402,11 +407,13 @@ __global__ __launch_bounds__(LZ_T) void lz_fin_ortho(const LanczosState *sts, co
 }
 
 // 8: HC -= sum_t v_t o_t;  partial ||HC||^2 for the next beta.  A thread
-// owns LZ_RR rows of each block step (rows j + r * LZ_T) and walks the basis
+// owns RR rows of each block step (rows j + r * LZ_T) and walks the basis
 // vectors with one load per row in flight, two vectors at a time, so every
-// lane keeps 2 * LZ_RR independent 16-byte loads outstanding.  The o_t are
-// the same for every lane: they are read as uniform (scalar-cache) loads.
-#define LZ_RR 4
+// lane keeps 2 * RR independent 16-byte loads outstanding.  The o_t are the
+// same for every lane: they are read as uniform (scalar-cache) loads.  RR
+// is 4, or 2 when the chain's instances have too few rows for 4 to fill
+// the device (update_rows).
+template <int RR>
 __global__ __launch_bounds__(LZ_T) void lz_reorth_norm(const LanczosState *sts, const LzInst *in)
 {
     const LanczosState *st = sts + blockIdx.y;
@@ -418,11 +425,11 @@ __global__ __launch_bounds__(LZ_T) void lz_reorth_norm(const LanczosState *sts, 
     const LZ_GLOBAL cplx *V = gp(I.lvec);
     LZ_GLOBAL cplx *HC = gp(I.HC);
     double s = 0;
-    for (int64_t base = (int64_t)blockIdx.x * LZ_T * LZ_RR + threadIdx.x; base < n;
-         base += (int64_t)I.nb * LZ_T * LZ_RR) {
-        cplx h[LZ_RR];
+    for (int64_t base = (int64_t)blockIdx.x * LZ_T * RR + threadIdx.x; base < n;
+         base += (int64_t)I.nb * LZ_T * RR) {
+        cplx h[RR];
 #pragma unroll
-        for (int r = 0; r < LZ_RR; r++) {
+        for (int r = 0; r < RR; r++) {
             const int64_t j = base + (int64_t)r * LZ_T;
             h[r] = (j < n) ? ldv(HC + j) : cmake(0, 0);
         }
@@ -431,7 +438,7 @@ __global__ __launch_bounds__(LZ_T) void lz_reorth_norm(const LanczosState *sts, 
             const cplx ot = cmake(o[tt].re, o[tt].im);
             const LZ_GLOBAL cplx *vt = V + (int64_t)tt * n;
 #pragma unroll
-            for (int r = 0; r < LZ_RR; r++) {
+            for (int r = 0; r < RR; r++) {
                 const int64_t j = base + (int64_t)r * LZ_T;
                 if (j < n) {
                     const cplx x = ldv_nt(vt + j);
@@ -441,7 +448,7 @@ __global__ __launch_bounds__(LZ_T) void lz_reorth_norm(const LanczosState *sts, 
             }
         }
 #pragma unroll
-        for (int r = 0; r < LZ_RR; r++) {
+        for (int r = 0; r < RR; r++) {
             const int64_t j = base + (int64_t)r * LZ_T;
             if (j < n) {
                 stv(HC + j, h[r]);
@@ -675,15 +682,29 @@ mcpi_status tridiag_lowest(int n, const double *d, const double *e, double *lamb
 
 // ---------------------------------------------------------------- host --
 
-// the Gram-Schmidt update of an instance with fewer than 256 row blocks is
-// split over vector groups (lz_reorth_part / lz_reorth_sum) when there are
-// more vectors than one group
-static bool split_update(int nb, int lm) { return nb < 256 && lm > LZ_TG; }
-
-static int vec_blocks(int64_t n)
+// blocks of LZ_T threads with rr rows each that cover n rows (at least one;
+// above the cap the vector kernels stride)
+static int64_t row_blocks(int64_t n, int rr)
 {
-    int64_t b = (n + LZ_T * 4 - 1) / (LZ_T * 4);
-    return (int)std::max<int64_t>(1, std::min<int64_t>(b, 1024));
+    return std::max<int64_t>(1, std::min<int64_t>((n + (int64_t)LZ_T * rr - 1) / ((int64_t)LZ_T * rr), 8192));
+}
+
+// the Gram-Schmidt update of an instance with fewer than 256 blocks of 4
+// rows per thread is split over vector groups (lz_reorth_part /
+// lz_reorth_sum) when there are more vectors than one group
+static bool split_update(int64_t n, int lm) { return row_blocks(n, 4) < 256 && lm > LZ_TG; }
+
+// rows per thread of the single-pass update for a chain of instances: 2
+// when 4 leave fewer than 1024 blocks in all (4 per CU), else 4.  Measured
+// on MI355X over steps 4..121: one instance of 646,646 rows 6.7 TB/s with 2
+// (632 blocks of 4 rows: 4.3), 18 of 43,758 rows 6.2 (4.6); one of
+// 1,352,078 rows 6.5 with 4 (2 rows, 2,641 blocks: 5.6), 24 of 646,646
+// rows 6.0 with 4 (5.9).
+static int update_rows(const std::vector<int64_t> &ns)
+{
+    int64_t b4 = 0;
+    for (int64_t n : ns) b4 += row_blocks(n, 4);
+    return b4 < 1024 ? 2 : 4;
 }
 
 // One device allocation carved into aligned pieces.
@@ -916,11 +937,14 @@ static mcpi_status lanczos_core(int K, const mcpi_ham *hams, const cplx *const *
     A.reserve(sizeof(LzInst) * K);
     std::vector<LzInst> hin(K);
     int nbmax = 1, nbgmax = 1;
+    std::vector<int64_t> ns(K);
+    for (int x = 0; x < K; x++) ns[x] = hams[x]->basis->d.nc;
+    const int rr = update_rows(ns);
     for (int x = 0; x < K; x++) {
-        const int64_t n = hams[x]->basis->d.nc;
+        const int64_t n = ns[x];
         LzInst &I = hin[x];
         I.n = n;
-        I.nb = vec_blocks(n);
+        I.nb = (int)row_blocks(n, rr);
         I.nbg = (int)((n + LZ_SEG - 1) / LZ_SEG);
         nbmax = std::max(nbmax, I.nb);
         nbgmax = std::max(nbgmax, I.nbg);
@@ -931,7 +955,7 @@ static mcpi_status lanczos_core(int K, const mcpi_ham *hams, const cplx *const *
         A.reserve(sizeof(double) * 2 * I.nb);
         A.reserve(sizeof(double) * 2 * (size_t)I.nbg * lm);
         A.reserve(sizeof(double) * lm);
-        if (split_update(I.nb, lm)) A.reserve(sizeof(cplx) * n * (size_t)((lm + LZ_TG - 1) / LZ_TG));
+        if (split_update(n, lm)) A.reserve(sizeof(cplx) * n * (size_t)((lm + LZ_TG - 1) / LZ_TG));
     }
     for (HvGroup &g : groups) {
         if (!g.batched) continue;
@@ -975,7 +999,7 @@ static mcpi_status lanczos_core(int K, const mcpi_ham *hams, const cplx *const *
         I.partial_o = A.take<double>(2 * (size_t)I.nbg * lm);
         double *z = A.take<double>(lm);
         I.z = z;
-        I.rpart = split_update(I.nb, lm) ? A.take<cplx>(n * (size_t)((lm + LZ_TG - 1) / LZ_TG)) : nullptr;
+        I.rpart = split_update(n, lm) ? A.take<cplx>(n * (size_t)((lm + LZ_TG - 1) / LZ_TG)) : nullptr;
         any_split = any_split || I.rpart != nullptr;
         I.Cout = (Cout && Cout[x]) ? Cout[x] : nullptr;
         I.iperm = hams[x]->basis->iperm;
@@ -1004,7 +1028,7 @@ static mcpi_status lanczos_core(int K, const mcpi_ham *hams, const cplx *const *
     MCPI_HIP(hipMemcpyAsync(din, hin.data(), sizeof(LzInst) * K, hipMemcpyHostToDevice, s));
 
     const dim3 blk(LZ_T), gv(nbmax, K), g1(1, K);
-    const dim3 gd(nbgmax, (lm + LZ_TG - 1) / LZ_TG, K);
+    const dim3 gd(nbgmax, (lm + LZ_DTG - 1) / LZ_DTG, K), gpart(nbgmax, (lm + LZ_TG - 1) / LZ_TG, K);
     const dim3 go(lm, K);
     const int ga = (K + 63) / 64;
 
@@ -1033,11 +1057,14 @@ static mcpi_status lanczos_core(int K, const mcpi_ham *hams, const cplx *const *
             hipLaunchKernelGGL(lz_sub_dot, gv, blk, 0, s, cst, cin);
             hipLaunchKernelGGL(lz_fin_alpha, g1, blk, 0, s, sts, cin, 0);
             hipLaunchKernelGGL(lz_sub_alpha, gv, blk, 0, s, cst, cin);
-            hipLaunchKernelGGL(lz_gemv_dots, gd, blk, 0, s, cst, cin, lm);
+            hipLaunchKernelGGL(lz_gemv_dots, gd, blk, 0, s, cst, cin);
             hipLaunchKernelGGL(lz_fin_ortho, go, blk, 0, s, cst, cin);
-            hipLaunchKernelGGL(lz_reorth_norm, gv, blk, 0, s, cst, cin);
+            if (rr == 2)
+                hipLaunchKernelGGL(lz_reorth_norm<2>, gv, blk, 0, s, cst, cin);
+            else
+                hipLaunchKernelGGL(lz_reorth_norm<4>, gv, blk, 0, s, cst, cin);
             if (any_split) {
-                hipLaunchKernelGGL(lz_reorth_part, gd, blk, 0, s, cst, cin);
+                hipLaunchKernelGGL(lz_reorth_part, gpart, blk, 0, s, cst, cin);
                 hipLaunchKernelGGL(lz_reorth_sum, gv, blk, 0, s, cst, cin);
             }
             hipLaunchKernelGGL(lz_advance, dim3(ga), dim3(64), 0, s, sts, K);
