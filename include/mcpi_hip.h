@@ -401,7 +401,11 @@ mcpi_status mcpi_hip_lanczos_batched(int n, const mcpi_ham *hams,
  * timing the candidates on the first step (default); 1 = one launch of the
  * specialised kernel over the group; 2 = the multi-vector kernel (one pass
  * over the rows for up to four instances that share the zero pattern of
- * their coefficients), where the group allows it.  Process-wide. */
+ * their coefficients), where the group allows it; 3 = the factorized pair
+ * form (schedule 7) four members at a time, where every member has it with
+ * the same blocks and b factors (Lieb-Liniger couplings of one basis), the
+ * timed choice otherwise.  The timed choice (0) includes that candidate.
+ * Process-wide. */
 mcpi_status mcpi_hip_set_batched_kernel(int kind);
 
 #ifdef __cplusplus

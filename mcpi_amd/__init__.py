@@ -533,7 +533,8 @@ def lanczos_batched(hams, starts, lm: int, outs=None):
 
 def set_batched_kernel(kind: int) -> None:
     """H.v kernel of batched groups: 0 timed (default), 1 batched specialised
-    kernel, 2 multi-vector kernel where allowed (mcpi_hip_set_batched_kernel)."""
+    kernel, 2 multi-vector kernel where allowed, 3 the factorized pair form
+    four members at a time where allowed (mcpi_hip_set_batched_kernel)."""
     _check(lib().mcpi_hip_set_batched_kernel(int(kind)), "set_batched_kernel")
 
 

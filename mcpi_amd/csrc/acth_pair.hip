@@ -56,6 +56,8 @@ struct PairArgs {
     const int32_t *zord;  // factorized form: pairs block by block (W = zb of the pairs)
     const int32_t *zoff;  // block r at zord[zoff[r] .. zoff[r+1])
     int zR;               // blocks
+    int64_t mC, mY;       // factorized form, several members: entries between
+                          // the members' C vectors and between their Z tables
 };
 
 __device__ inline cplx ldcp(const cplx *p)
@@ -223,10 +225,13 @@ struct PairAmpLds {
 
 // FAC: instead of X, Z[r][u] = sum_{Q in block r} zb[Q] X[Q][u] for the
 // R blocks of the factorized pair form (W = zb; a wavefront per block and
-// 64 rows); Z has R rows of ld entries.
-template <int M, bool FAC>
+// 64 rows); Z has R rows of ld entries.  MV > 1 (FAC only): the same for MV
+// members that share zb and the blocks, their C vectors mC and their Z
+// tables mY entries apart (the unranking and the targets are shared).
+template <int M, bool FAC, int MV = 1>
 __global__ __launch_bounds__(PT) void k_pair_amp(PairArgs a)
 {
+    static_assert(MV == 1 || FAC, "several members: factorized form only");
     constexpr int NP = pair_count(M), U = PairAmpLds<M>::U;
     extern __shared__ __align__(16) unsigned char lds[];
     int64_t *tt = (int64_t *)lds;                              // [U]
@@ -292,7 +297,9 @@ __global__ __launch_bounds__(PT) void k_pair_amp(PairArgs a)
                 const int64_t base = tt[ul];
                 if (base < 0) continue;
                 const int j0 = a.zoff[r], j1 = a.zoff[r + 1];
-                cplx acc = cmake(0, 0);
+                cplx acc[MV];
+#pragma unroll
+                for (int m = 0; m < MV; m++) acc[m] = cmake(0, 0);
                 // the block's pairs two at a time: a small register count
                 // (many wavefronts per SIMD) hides the gathers better than
                 // more loads per thread (four: +4% at N=16/M=12, eight and
@@ -301,7 +308,7 @@ __global__ __launch_bounds__(PT) void k_pair_amp(PairArgs a)
                 for (int j = j0; j < j1; j += GBK) {
                     int64_t tg[GBK];
                     double f[GBK];
-                    cplx w[GBK], v[GBK];
+                    cplx w[GBK], v[MV][GBK];
 #pragma unroll
                     for (int k = 0; k < GBK; k++) {
                         tg[k] = -1;
@@ -314,14 +321,22 @@ __global__ __launch_bounds__(PT) void k_pair_amp(PairArgs a)
                         }
                     }
 #pragma unroll
-                    for (int k = 0; k < GBK; k++) v[k] = tg[k] >= 0 ? ldcp(a.C + tg[k]) : cmake(0, 0);
+                    for (int m = 0; m < MV; m++)
 #pragma unroll
-                    for (int k = 0; k < GBK; k++) acc = cfma(cscale(w[k], f[k]), v[k], acc);
+                        for (int k = 0; k < GBK; k++)
+                            v[m][k] = tg[k] >= 0 ? ldcp(a.C + m * a.mC + tg[k]) : cmake(0, 0);
+#pragma unroll
+                    for (int m = 0; m < MV; m++)
+#pragma unroll
+                        for (int k = 0; k < GBK; k++) acc[m] = cfma(cscale(w[k], f[k]), v[m][k], acc[m]);
                 }
-                double2 o;
-                o.x = acc.re;
-                o.y = acc.im;
-                *reinterpret_cast<double2 *>(a.Y + (int64_t)r * a.ld + u0 + ul) = o;
+#pragma unroll
+                for (int m = 0; m < MV; m++) {
+                    double2 o;
+                    o.x = acc[m].re;
+                    o.y = acc[m].im;
+                    *reinterpret_cast<double2 *>(a.Y + m * a.mY + (int64_t)r * a.ld + u0 + ul) = o;
+                }
             }
             continue;
         }
@@ -651,6 +666,111 @@ __global__ __launch_bounds__(PT) void k_pair_out(int N, int64_t nc, const int32_
     *reinterpret_cast<double2 *>(out + i) = o;
 }
 
+// The output pass of the factorized form for MV members that share the
+// basis, zb and the blocks (a group of Lieb-Liniger couplings): the row's
+// occupations, prefix sums, pair offsets and factors are computed once and
+// serve every member; member m reads its C, Z, za and h1 at C + m mC,
+// Z + m mZ and (bytes) za + m mK, h1 + m mK, and writes out + m mC.
+// Batches of BT terms for all members (MV BT loads in flight).
+template <int M, int MV>
+__global__ __launch_bounds__(PT) void k_pairz_out_mv(int N, int64_t nc, const int32_t *__restrict__ IF,
+                                                     const int32_t *__restrict__ Bg, int64_t nu2, int64_t ld2,
+                                                     const cplx *__restrict__ Z, int64_t mZ,
+                                                     const cplx *__restrict__ za, const int32_t *__restrict__ zblk,
+                                                     const cplx *__restrict__ h1, int64_t mK,
+                                                     const cplx *__restrict__ C, cplx *__restrict__ out, int64_t mC)
+{
+    extern __shared__ __align__(16) unsigned char lds[];
+    double *sq = (double *)lds;                      // [N + 3]
+    int32_t *Bt = (int32_t *)(sq + N + 3);          // [M][N + 1]
+    const int NB = N + 1;
+    for (int x = threadIdx.x; x < N + 3; x += PT) sq[x] = sqrt((double)x);
+    for (int x = threadIdx.x; x < M * NB; x += PT) Bt[x] = Bg[x];
+    __syncthreads();
+    const int64_t i = (int64_t)blockIdx.x * PT + threadIdx.x;
+    if (i >= nc) return;
+    int v[M];
+#pragma unroll
+    for (int k = 0; k < M; k++) v[k] = IF[i * M + k];
+    uint32_t S0[M], S1[M], S2[M];
+    {
+        int pj = 0;
+        uint32_t s0 = 0, s1 = 0, s2 = 0;
+        S0[0] = S1[0] = S2[0] = 0;
+#pragma unroll
+        for (int j = 1; j < M; j++) {
+            pj += v[j - 1];
+            const int b = j * NB + pj;
+            s0 += (uint32_t)Bt[b];
+            s1 += (uint32_t)Bt[pj >= 1 ? b - 1 : b];
+            s2 += (uint32_t)Bt[pj >= 2 ? b - 2 : b];
+            S0[j] = s0;
+            S1[j] = s1;
+            S2[j] = s2;
+        }
+    }
+    constexpr uint32_t OUT = 0x80000000u;
+    auto mza = [&](int m) { return (const cplx *)((const char *)za + m * mK); };
+    cplx acc[MV];
+#pragma unroll
+    for (int m = 0; m < MV; m++) {   // the diagonal one-body part
+        const cplx *h = (const cplx *)((const char *)h1 + m * mK);
+        cplx d = cmake(0, 0);
+#pragma unroll
+        for (int k = 0; k < M; k++) {
+            const cplx hk = ldcp(h + k * (pair_h1_stride(M) + 1));
+            d.re = fma(hk.re, (double)v[k], d.re);
+            d.im = fma(hk.im, (double)v[k], d.im);
+        }
+        acc[m] = cfma(d, ldcp(C + m * mC + i), cmake(0, 0));
+    }
+    const uint32_t top2 = (uint32_t)(nu2 - 1) - S2[M - 1];
+    const int bytes2 = (int)(nu2 * (int64_t)sizeof(cplx));
+    constexpr int NPp = pair_count(M), BT = MV >= 4 ? 4 : 6;
+#pragma clang loop unroll(full)
+    for (int b0 = 0; b0 < NPp; b0 += BT) {
+        uint32_t off[BT];
+        int bk[BT];
+        double f[BT];
+        cplx y[MV][BT];
+#pragma unroll
+        for (int j = 0; j < BT; j++) {
+            const int P = b0 + j;
+            if (P >= NPp) break;
+            const int c2 = pair_hi(P), c1 = P - c2 * (c2 + 1) / 2;
+            const int blk = zblk[P];
+            const bool occ = (c1 < c2 ? (v[c1] >= 1 && v[c2] >= 1) : v[c1] >= 2) && blk >= 0;
+            off[j] = occ ? (top2 - (S0[c1] - S1[c1]) - (S1[c2] - S2[c2])) * (uint32_t)sizeof(cplx) : OUT;
+            f[j] = c1 < c2 ? sq[v[c1]] * sq[v[c2]] : sq[v[c1]] * sq[v[c1] >= 1 ? v[c1] - 1 : 0];
+            bk[j] = blk >= 0 ? blk : 0;
+        }
+#pragma unroll
+        for (int m = 0; m < MV; m++)
+#pragma unroll
+            for (int j = 0; j < BT; j++) {
+                if (b0 + j >= NPp) break;
+                y[m][j] = ldbuf(Z + m * mZ + (int64_t)bk[j] * ld2, bytes2, off[j]);
+            }
+#pragma unroll
+        for (int m = 0; m < MV; m++)
+#pragma unroll
+            for (int j = 0; j < BT; j++) {
+                if (b0 + j >= NPp) break;
+                acc[m] = cfma(cscale(ldcp(mza(m) + b0 + j), f[j]), y[m][j], acc[m]);
+            }
+        // each batch summed before the next one's loads issue
+#pragma unroll
+        for (int m = 0; m < MV; m++) asm volatile("" : "+v"(acc[m].re), "+v"(acc[m].im) : : "memory");
+    }
+#pragma unroll
+    for (int m = 0; m < MV; m++) {
+        double2 o;
+        o.x = acc[m].re;
+        o.y = acc[m].im;
+        *reinterpret_cast<double2 *>(out + m * mC + i) = o;
+    }
+}
+
 template <int M, int U>
 mcpi_status launch_y1(const PairArgs &a, int ncu, hipStream_t st)
 {
@@ -749,6 +869,35 @@ mcpi_status launch_pairz_m(const BasisDev &b, const HamCoefDev &c, const cplx *C
     return MCPI_OK;
 }
 
+// the factorized pair form for MV members at once (member m's Z at
+// ws + m * R * ld2; vectors mC entries and coefficient blocks mK bytes
+// apart, member 0's tables in c)
+template <int M, int MV>
+mcpi_status launch_pairz_mv_m(const BasisDev &b, const HamCoefDev &c, const cplx *C, cplx *out, int64_t mC,
+                              int64_t mK, cplx *ws, hipStream_t st)
+{
+    int dev = 0, ncu = 0;
+    MCPI_HIP(hipGetDevice(&dev));
+    MCPI_HIP(hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev));
+    const int N = b.N;
+    const int64_t nu2 = nc_of(N - 2, M), ld2 = pair_ld(nu2), mZ = (int64_t)c.pzR * ld2;
+    constexpr int U = PairAmpLds<M>::U;
+    PairArgs a{N,   b.nc,    nu2,     (nu2 + U - 1) / U, b.B, C, c.pzb, ws, ld2, nullptr, 0.0,
+               c.pzord, c.pzoff, c.pzR, mC,                mZ};
+    const size_t sm = PairAmpLds<M>::bytes(N);
+    int per_cu = 0;
+    MCPI_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, (const void *)k_pair_amp<M, true, MV>, PT, sm));
+    const int64_t nb = std::max<int64_t>(1, std::min<int64_t>(a.nut, (int64_t)ncu * std::max(per_cu, 1)));
+    hipLaunchKernelGGL((k_pair_amp<M, true, MV>), dim3((unsigned)nb), dim3(PT), sm, st, a);
+    MCPI_HIP(hipGetLastError());
+    const size_t so = sizeof(double) * (N + 3) + sizeof(int32_t) * M * (N + 1);
+    hipLaunchKernelGGL((k_pairz_out_mv<M, MV>), dim3((unsigned)((b.nc + PT - 1) / PT)), dim3(PT), so, st, N, b.nc,
+                       (const int32_t *)b.IF, (const int32_t *)b.B, nu2, ld2, (const cplx *)ws, mZ,
+                       (const cplx *)c.pza, (const int32_t *)c.pzblk, (const cplx *)c.ph1, mK, C, out, mC);
+    MCPI_HIP(hipGetLastError());
+    return MCPI_OK;
+}
+
 }  // namespace
 
 // complex entries of the pair-space workspace of a basis: Y1 (M rows of
@@ -824,6 +973,23 @@ mcpi_status launch_pairz(const BasisDev &b, const HamCoefDev &c, const cplx *C, 
     case 24: return launch_pairz_m<24>(b, c, C, out, ws, st);
     default: return MCPI_ERR_ARG;
     }
+}
+
+mcpi_status launch_pairz_multi(const BasisDev &b, const HamCoefDev &c, int mv, const cplx *C, cplx *out, int64_t mC,
+                               int64_t mK, cplx *ws, hipStream_t st)
+{
+    MCPI_ARG(pairz_applicable(b, c) && b.M <= MCPI_PAIRZ_MULTI_MAXM && (mv == 2 || mv == 4),
+             "the factorized pair form for several members does not apply");
+    MCPI_ARG(mK % 16 == 0 && mC >= b.nc, "member strides");
+#define MCPI_PZMV(MM) \
+    case MM: return mv == 4 ? launch_pairz_mv_m<MM, 4>(b, c, C, out, mC, mK, ws, st) \
+                            : launch_pairz_mv_m<MM, 2>(b, c, C, out, mC, mK, ws, st);
+    switch (b.M) {
+        MCPI_PZMV(3) MCPI_PZMV(4) MCPI_PZMV(5) MCPI_PZMV(6) MCPI_PZMV(7) MCPI_PZMV(8) MCPI_PZMV(9) MCPI_PZMV(10)
+        MCPI_PZMV(11) MCPI_PZMV(12) MCPI_PZMV(13)
+    default: return MCPI_ERR_ARG;
+    }
+#undef MCPI_PZMV
 }
 
 }  // namespace mcpi

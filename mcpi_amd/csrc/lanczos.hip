@@ -752,14 +752,73 @@ struct HvGroup {
     std::vector<int> members;
     bool batched = false;
     bool multi_ok = false;        // members share the zero pattern: multi-vector kernel allowed
+    bool pzmv_ok = false;         // the factorized pair form several members at a time allowed
     int mode = 1;                 // 1 batched specialised kernel, 2 multi-vector kernel,
-                                  // 3 each member on its own with its tuned schedule
+                                  // 3 each member on its own with its tuned schedule,
+                                  // 4 the factorized pair form four members at a time
     int sched = 2, skip = 0;
     size_t cbytes = 0;            // coefficient block size = stride
     HamCoefDev c0{};              // member 0's tables inside `coefs`
     char *coefs = nullptr;        // members' coefficient blocks, cbytes apart
     cplx *vin = nullptr, *HC = nullptr;   // members' vectors, nc apart
+    cplx *zws = nullptr;          // mode 4: Z tables of four members
 };
+
+// mode 4 applies to a batched group when every member has the factorized
+// pair form with the same blocks and b factors (Lieb-Liniger couplings of
+// one basis: only a and the diagonal one-body part differ) and its tables
+// at the same offsets of its coefficient block
+static bool pairz_multi_ok(const HvGroup &g, const mcpi_ham *hams)
+{
+    const BasisDev &b = g.basis->d;
+    const mcpi_ham h0 = hams[g.members[0]];
+    const HamCoefDev &c0 = h0->c;
+    if (g.basis->sectored() || b.M > MCPI_PAIRZ_MULTI_MAXM || !pairz_applicable(b, c0) || h0->coef_bytes % 16)
+        return false;
+    const int NP = pair_count(b.M), R = c0.pzR;
+    auto off = [](const mcpi_ham h, const void *p) { return (const char *)p - (const char *)h->coef_block; };
+    auto tables = [&](const HamCoefDev &c, std::vector<char> &t) {
+        t.resize(sizeof(cplx) * NP + sizeof(int32_t) * (2 * NP + R + 1));
+        char *p = t.data();
+        if (hipMemcpy(p, c.pzb, sizeof(cplx) * NP, hipMemcpyDeviceToHost) != hipSuccess ||
+            hipMemcpy(p + sizeof(cplx) * NP, c.pzblk, sizeof(int32_t) * NP, hipMemcpyDeviceToHost) != hipSuccess ||
+            hipMemcpy(p + sizeof(cplx) * NP + sizeof(int32_t) * NP, c.pzoff, sizeof(int32_t) * (R + 1),
+                      hipMemcpyDeviceToHost) != hipSuccess)
+            return false;
+        const int32_t *zoff = (const int32_t *)(p + sizeof(cplx) * NP + sizeof(int32_t) * NP);
+        const int nord = zoff[R];
+        if (nord < 0 || nord > NP) return false;
+        return hipMemcpy(p + sizeof(cplx) * NP + sizeof(int32_t) * (NP + R + 1), c.pzord, sizeof(int32_t) * nord,
+                         hipMemcpyDeviceToHost) == hipSuccess;
+    };
+    std::vector<char> t0, t;
+    if (!tables(c0, t0)) {
+        (void)hipGetLastError();
+        return false;
+    }
+    for (int x : g.members) {
+        const mcpi_ham h = hams[x];
+        const HamCoefDev &c = h->c;
+        if (!pairz_applicable(b, c) || c.pzR != R || h->coef_bytes != h0->coef_bytes ||
+            off(h, c.pza) != off(h0, c0.pza) || off(h, c.pzb) != off(h0, c0.pzb) ||
+            off(h, c.pzblk) != off(h0, c0.pzblk) || off(h, c.pzord) != off(h0, c0.pzord) ||
+            off(h, c.pzoff) != off(h0, c0.pzoff) || off(h, c.ph1) != off(h0, c0.ph1))
+            return false;
+        if (x == g.members[0]) continue;
+        if (!tables(c, t)) {
+            (void)hipGetLastError();
+            return false;
+        }
+        if (t != t0) return false;
+    }
+    return true;
+}
+
+static size_t pairz_multi_bytes(const HvGroup &g, const mcpi_ham *hams)
+{
+    const BasisDev &b = g.basis->d;
+    return sizeof(cplx) * 4 * (size_t)hams[g.members[0]]->c.pzR * (size_t)pair_ld(nc_of(b.N - 2, b.M));
+}
 
 static mcpi_status launch_group(const HvGroup &g, const mcpi_ham *hams, const std::vector<LzInst> &hin,
                                 hipStream_t s)
@@ -769,6 +828,24 @@ static mcpi_status launch_group(const HvGroup &g, const mcpi_ham *hams, const st
         // pair form where it applies); the members share the basis's pair
         // workspace on this stream one after the other
         for (int x : g.members) MCPI_CHECK(launch_apply(hams[x], hin[x].vin, hin[x].HC, s));
+        return MCPI_OK;
+    }
+    if (g.batched && g.mode == 4) {
+        // the factorized pair form four members at a time (then two, then
+        // one) from the group's copies of the coefficient blocks
+        const BasisDev &bd = g.basis->d;
+        const int m = (int)g.members.size();
+        int k = 0;
+        while (k < m) {
+            const int mv = m - k >= 4 ? 4 : (m - k >= 2 ? 2 : 1);
+            const HamCoefDev ck = shift_coef(g.c0, (ptrdiff_t)(k * g.cbytes));
+            const size_t o = (size_t)k * (size_t)bd.nc;
+            if (mv == 1)
+                MCPI_CHECK(launch_pairz(bd, ck, g.vin + o, g.HC + o, g.zws, s));
+            else
+                MCPI_CHECK(launch_pairz_multi(bd, ck, mv, g.vin + o, g.HC + o, bd.nc, (int64_t)g.cbytes, g.zws, s));
+            k += mv;
+        }
         return MCPI_OK;
     }
     if (g.batched) {
@@ -805,12 +882,14 @@ static mcpi_status launch_group(const HvGroup &g, const mcpi_ham *hams, const st
 // pair form applies, also the members one by one with their own tuned H.v.
 // fixed: a schedule the caller set on the Hamiltonians (0: free).
 // kernel choice for batched groups: 0 = timed, 1 = batched specialised
-// kernel only, 2 = multi-vector kernel where the group allows it
+// kernel only, 2 = multi-vector kernel where the group allows it, 3 = the
+// factorized pair form several members at a time where the group allows
+// it (timed otherwise)
 static std::atomic<int> g_batched_kernel{0};
 
 mcpi_status set_batched_kernel(int kind)
 {
-    MCPI_ARG(kind >= 0 && kind <= 2, "kind must be 0, 1 or 2");
+    MCPI_ARG(kind >= 0 && kind <= 3, "kind must be 0, 1, 2 or 3");
     g_batched_kernel = kind;
     return MCPI_OK;
 }
@@ -828,6 +907,11 @@ static mcpi_status tune_group(HvGroup &g, const mcpi_ham *hams, const std::vecto
         }
         fixed = 4;
     }
+    if (g_batched_kernel == 3 && g.pzmv_ok && !fixed) {
+        g.mode = 4;
+        g.sched = 0;
+        return MCPI_OK;
+    }
     for (int mode = 1; mode <= 2; mode++) {
         if (mode == 2 && !g.multi_ok) continue;
         if (g_batched_kernel == 1 && mode == 2) continue;
@@ -844,9 +928,11 @@ static mcpi_status tune_group(HvGroup &g, const mcpi_ham *hams, const std::vecto
     if (cand.empty()) cand.push_back({1, fixed ? fixed : 2});
     // the members one by one, each with its own tuned H.v: pays one launch
     // pair per member, and wins where the factorized pair form applies
-    if (!fixed && g_batched_kernel == 0 && !g.basis->sectored() &&
+    if (!fixed && (g_batched_kernel == 0 || g_batched_kernel == 3) && !g.basis->sectored() &&
         pairz_applicable(g.basis->d, hams[g.members[0]]->c))
         cand.push_back({3, 0});
+    // the same four members at a time: their row and pair arithmetic shared
+    if (!fixed && (g_batched_kernel == 0 || g_batched_kernel == 3) && g.pzmv_ok) cand.push_back({4, 0});
     if (cand.size() == 1) {
         g.mode = cand[0].first;
         g.sched = cand[0].second;
@@ -927,6 +1013,7 @@ static mcpi_status lanczos_core(int K, const mcpi_ham *hams, const cplx *const *
             g.multi_ok = g.skip && same_zeros && g.basis->d.Map != nullptr;
             g.cbytes = h0->coef_bytes;
             g.c0.real = real ? 1 : 0;
+            g.pzmv_ok = pairz_multi_ok(g, hams);
             for (size_t k = 0; k < g.members.size(); k++) batched_member[g.members[k]] = (int)k;
         }
     }
@@ -962,6 +1049,7 @@ static mcpi_status lanczos_core(int K, const mcpi_ham *hams, const cplx *const *
         const size_t m = g.members.size();
         A.reserve(g.cbytes * m);
         for (int k = 0; k < 2; k++) A.reserve(sizeof(cplx) * (size_t)g.basis->d.nc * m);
+        if (g.pzmv_ok) A.reserve(pairz_multi_bytes(g, hams));
     }
     MCPI_CHECK(A.allocate());
     bool any_split = false;
@@ -973,6 +1061,7 @@ static mcpi_status lanczos_core(int K, const mcpi_ham *hams, const cplx *const *
         g.coefs = A.take<char>(g.cbytes * m);
         g.vin = A.take<cplx>((size_t)g.basis->d.nc * m);
         g.HC = A.take<cplx>((size_t)g.basis->d.nc * m);
+        if (g.pzmv_ok) g.zws = (cplx *)A.take<char>(pairz_multi_bytes(g, hams));
         if (A.overflow) break;
         for (size_t k = 0; k < m; k++) {
             const mcpi_ham h = hams[g.members[k]];

@@ -304,6 +304,9 @@ __host__ __device__ inline int pair_of2(int a, int b)
 #define MCPI_PAIR_MAXM 12
 // the factorized pair form (schedule 7) has no product in LDS: larger M
 #define MCPI_PAIRZ_MAXM 24
+// its several-member launch (batched groups) covers the M of the
+// specialised kernels
+#define MCPI_PAIRZ_MULTI_MAXM 13
 
 struct mcpi_ham_s {
     mcpi_basis basis;
@@ -374,6 +377,13 @@ mcpi_status launch_pair(const BasisDev &b, const HamCoefDev &c, const cplx *C, c
 bool pairz_applicable(const BasisDev &b, const HamCoefDev &c);
 mcpi_status launch_pairz(const BasisDev &b, const HamCoefDev &c, const cplx *C, cplx *out, cplx *ws,
                          hipStream_t st);
+// the same for mv (2 or 4) Hamiltonians of one basis whose pair matrices
+// share the blocks and b factors (M <= MCPI_PAIRZ_MULTI_MAXM): member m's
+// vectors at C + m mC and out + m mC, its coefficient block mK bytes after
+// member m - 1's (member 0's tables in c); a workspace of
+// mv x (blocks) x pair_ld(NC(N-2, M)) complex entries
+mcpi_status launch_pairz_multi(const BasisDev &b, const HamCoefDev &c, int mv, const cplx *C, cplx *out, int64_t mC,
+                               int64_t mK, cplx *ws, hipStream_t st);
 // build the requested maps of a basis whose IF table is in place
 mcpi_status basis_build_parts(mcpi_basis b, int want_map, int want_ot, int want_tt);
 

@@ -24,7 +24,7 @@ def _ll_instances(spec, seed):
     return [(N, M, float(10 ** rng.uniform(-1, 2))) for (N, M) in spec]
 
 
-@pytest.fixture(params=[0, 1, 2], ids=["timed", "batched_kernel", "multi_vector"])
+@pytest.fixture(params=[0, 1, 2, 3], ids=["timed", "batched_kernel", "multi_vector", "factorized_multi"])
 def batched_kernel(request):
     mcpi_amd.set_batched_kernel(request.param)
     yield request.param
@@ -157,3 +157,47 @@ def test_multi_vector_sparse_generic(oracle_lib, batched_kernel):
         E, steps = mcpi_amd.lanczos_batched(hams, starts, 40)
         for x in range(5):
             assert abs(E[x] - ref[x]) < 1e-10, (maps, x, E[x], ref[x])
+
+
+def test_factorized_pair_form_several_members(oracle_lib):
+    """Batched groups whose members all have the factorized pair form with
+    the same blocks (Lieb-Liniger couplings of one basis, a random diagonal
+    Ho per member) run it four members at a time, then two, then one: seven
+    members of (8,11) and three of (6,9), with kind 3 forcing that path.  A
+    random Hamiltonian in a third basis's group keeps it from applying
+    there.  Every E0 and Ritz vector equals the one-instance solver's (the
+    same arithmetic per member), two instances are also checked against the
+    oracle."""
+    rng = np.random.default_rng(77)
+    spec = [(8, 11)] * 7 + [(6, 9)] * 3 + [(5, 6)] * 2
+    lm = 60
+    bases = {nm: mcpi_amd.Basis(nm[0], nm[1], "full") for nm in set(spec)}
+    hams, starts, outs, mats = [], [], [], []
+    for x, (N, M) in enumerate(spec):
+        b = bases[(N, M)]
+        if (N, M) == (5, 6) and x == len(spec) - 1:
+            _, Ho, Hint = oracle.random_inputs(N, M, seed=5)
+        else:
+            Ho = np.diag(oracle.ll_ho(M).diagonal() + rng.uniform(-1, 1, M)).astype(complex)
+            Hint = oracle.ll_hint(M, float(10 ** rng.uniform(-1, 2)))
+        mats.append((Ho, Hint))
+        hams.append(mcpi_amd.Hamiltonian(b, Ho, Hint))
+        starts.append(mcpi_amd.DeviceArray.from_numpy(_start(b.nc, 300 + x)))
+        outs.append(mcpi_amd.DeviceArray(b.nc))
+    note(f"factorized pair form several members, {len(spec)} instances lm={lm}")
+    mcpi_amd.set_batched_kernel(3)
+    try:
+        E, steps = mcpi_amd.lanczos_batched(hams, starts, lm, outs)
+    finally:
+        mcpi_amd.set_batched_kernel(0)
+    for x in range(len(spec)):
+        single = mcpi_amd.DeviceArray(hams[x].basis.nc)
+        E1, _, st1 = hams[x].ground(lm, starts[x], out=single)
+        assert steps[x] == st1 == lm
+        assert abs(E[x] - E1) < 1e-10, (x, spec[x], E[x], E1)
+        assert abs(abs(np.vdot(single.numpy(), outs[x].numpy())) - 1) < 1e-8
+    for x in (0, 6):
+        N, M = spec[x]
+        E_o, psi_o, _ = oracle.Basis(N, M).ground(lm, starts[x].numpy(), *mats[x])
+        assert abs(E[x] - E_o) < 1e-10, (x, E[x], E_o)
+        assert abs(abs(np.vdot(psi_o, outs[x].numpy())) - 1) < 1e-8
