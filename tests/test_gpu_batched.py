@@ -162,14 +162,15 @@ def test_multi_vector_sparse_generic(oracle_lib, batched_kernel):
 def test_factorized_pair_form_several_members(oracle_lib):
     """Batched groups whose members all have the factorized pair form with
     the same blocks (Lieb-Liniger couplings of one basis, a random diagonal
-    Ho per member) run it four members at a time, then two, then one: seven
-    members of (8,11) and three of (6,9), with kind 3 forcing that path.  A
+    Ho per member) run it four members at a time, then two, then one:
+    fifteen members of (8,11) and three of (6,9), with kind 3 forcing that
+    path.  A
     random Hamiltonian in a third basis's group keeps it from applying
     there.  Every E0 and Ritz vector equals the one-instance solver's (the
     same arithmetic per member), two instances are also checked against the
     oracle."""
     rng = np.random.default_rng(77)
-    spec = [(8, 11)] * 7 + [(6, 9)] * 3 + [(5, 6)] * 2
+    spec = [(8, 11)] * 15 + [(6, 9)] * 3 + [(5, 6)] * 2
     lm = 60
     bases = {nm: mcpi_amd.Basis(nm[0], nm[1], "full") for nm in set(spec)}
     hams, starts, outs, mats = [], [], [], []
@@ -196,7 +197,7 @@ def test_factorized_pair_form_several_members(oracle_lib):
         assert steps[x] == st1 == lm
         assert abs(E[x] - E1) < 1e-10, (x, spec[x], E[x], E1)
         assert abs(abs(np.vdot(single.numpy(), outs[x].numpy())) - 1) < 1e-8
-    for x in (0, 6):
+    for x in (0, 14):
         N, M = spec[x]
         E_o, psi_o, _ = oracle.Basis(N, M).ground(lm, starts[x].numpy(), *mats[x])
         assert abs(E[x] - E_o) < 1e-10, (x, E[x], E_o)
