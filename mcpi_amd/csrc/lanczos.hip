@@ -318,26 +318,28 @@ __global__ __launch_bounds__(LZ_T) void lz_sub_alpha(const LanczosState *sts, co
 
 // 6b: partial o_t = <v_t, HC> for t = 0..i+1.  Block (s, g, instance) owns
 // row segment s (LZ_R rows per thread, kept in registers) and basis vectors
-// t in [g*LZ_DTG, (g+1)*LZ_DTG); waves reduce with shuffles and park their
-// sums in LDS, one barrier per block.  16 vectors per block rather than 32
-// give twice the blocks at a given step: over steps 4..121 on MI355X, 5.5
-// against 4.5 TB/s at 646,646 rows, 6.4 against 6.3 at 13.0 M rows or with
-// 24 instances of 646,646 rows side by side.
+// t in [g*TG, (g+1)*TG); waves reduce with shuffles and park their sums in
+// LDS, one barrier per block.  TG is 16, or 8 for a chain with few row
+// segments (dot_group): fewer vectors per block give more blocks at a given
+// step.  Over steps 4..121 on MI355X: 646,646 rows 5.9 / 5.5 / 4.5 TB/s
+// with 8 / 16 / 32 vectors per block, 18 instances of 43,758 rows
+// 5.6 / 5.1 / 4.7, 13.0 M rows 6.3 / 6.4 / 6.4; 352,716 rows 4.8 / 4.3
+// with 8 / 16.
 #define LZ_R 8
-#define LZ_DTG 16
 #define LZ_SEG ((int64_t)LZ_T * LZ_R)
 // vector groups of the split Gram-Schmidt update (small bases)
 #define LZ_TG 32
+template <int TG>
 __global__ __launch_bounds__(LZ_T) void lz_gemv_dots(const LanczosState *sts, const LzInst *in)
 {
-    __shared__ double red[LZ_DTG][LZ_T / 64][2];
+    __shared__ double red[TG][LZ_T / 64][2];
     const LanczosState *st = sts + blockIdx.z;
     const LzInst &I = in[blockIdx.z];
     if ((int)blockIdx.x >= I.nbg || st->broke) return;
     const int nt = st->it + 2;
-    const int t0 = blockIdx.y * LZ_DTG;
+    const int t0 = blockIdx.y * TG;
     if (t0 >= nt) return;
-    const int t1 = min(nt, t0 + LZ_DTG);
+    const int t1 = min(nt, t0 + TG);
     const int64_t n = I.n;
     const int64_t base = (int64_t)blockIdx.x * LZ_SEG + threadIdx.x;
     cplx h[LZ_R];
@@ -707,6 +709,15 @@ static int update_rows(const std::vector<int64_t> &ns)
     return b4 < 1024 ? 2 : 4;
 }
 
+// basis vectors per block of the dot pass for a chain: 8 below 4096 row
+// segments in all, else 16
+static int dot_group(const std::vector<int64_t> &ns)
+{
+    int64_t sg = 0;
+    for (int64_t n : ns) sg += (n + LZ_SEG - 1) / LZ_SEG;
+    return sg < 4096 ? 8 : 16;
+}
+
 // One device allocation carved into aligned pieces.
 struct Arena {
     char *base = nullptr;
@@ -1026,7 +1037,7 @@ static mcpi_status lanczos_core(int K, const mcpi_ham *hams, const cplx *const *
     int nbmax = 1, nbgmax = 1;
     std::vector<int64_t> ns(K);
     for (int x = 0; x < K; x++) ns[x] = hams[x]->basis->d.nc;
-    const int rr = update_rows(ns);
+    const int rr = update_rows(ns), dtg = dot_group(ns);
     for (int x = 0; x < K; x++) {
         const int64_t n = ns[x];
         LzInst &I = hin[x];
@@ -1117,7 +1128,7 @@ static mcpi_status lanczos_core(int K, const mcpi_ham *hams, const cplx *const *
     MCPI_HIP(hipMemcpyAsync(din, hin.data(), sizeof(LzInst) * K, hipMemcpyHostToDevice, s));
 
     const dim3 blk(LZ_T), gv(nbmax, K), g1(1, K);
-    const dim3 gd(nbgmax, (lm + LZ_DTG - 1) / LZ_DTG, K), gpart(nbgmax, (lm + LZ_TG - 1) / LZ_TG, K);
+    const dim3 gd(nbgmax, (lm + dtg - 1) / dtg, K), gpart(nbgmax, (lm + LZ_TG - 1) / LZ_TG, K);
     const dim3 go(lm, K);
     const int ga = (K + 63) / 64;
 
@@ -1146,7 +1157,10 @@ static mcpi_status lanczos_core(int K, const mcpi_ham *hams, const cplx *const *
             hipLaunchKernelGGL(lz_sub_dot, gv, blk, 0, s, cst, cin);
             hipLaunchKernelGGL(lz_fin_alpha, g1, blk, 0, s, sts, cin, 0);
             hipLaunchKernelGGL(lz_sub_alpha, gv, blk, 0, s, cst, cin);
-            hipLaunchKernelGGL(lz_gemv_dots, gd, blk, 0, s, cst, cin);
+            if (dtg == 8)
+                hipLaunchKernelGGL(lz_gemv_dots<8>, gd, blk, 0, s, cst, cin);
+            else
+                hipLaunchKernelGGL(lz_gemv_dots<16>, gd, blk, 0, s, cst, cin);
             hipLaunchKernelGGL(lz_fin_ortho, go, blk, 0, s, cst, cin);
             if (rr == 2)
                 hipLaunchKernelGGL(lz_reorth_norm<2>, gv, blk, 0, s, cst, cin);
