@@ -34,6 +34,8 @@
 #include <algorithm>
 #include <cmath>
 #include <cstring>
+#include <map>
+#include <mutex>
 #include <vector>
 
 #include "mcpi_internal.h"
@@ -978,6 +980,65 @@ static mcpi_status tune_group(HvGroup &g, const mcpi_ham *hams, const std::vecto
     return rc;
 }
 
+// The choice of tune_group, kept across calls for the same group: the
+// basis, the member handles with their schedule and zero-skipping settings,
+// the kernel kind and the fixed schedule.  A handle freed and reallocated at
+// the same address may map to another Hamiltonian, so a cached choice is
+// used only while it still applies to the group (every mode computes the
+// same H.v; only the time differs).  At most 256 groups are kept.
+struct TuneKey {
+    const void *basis;
+    std::vector<uintptr_t> members;   // handle, schedule, skip_zero per member
+    int kind, fixed;
+    bool operator<(const TuneKey &o) const
+    {
+        if (basis != o.basis) return basis < o.basis;
+        if (kind != o.kind) return kind < o.kind;
+        if (fixed != o.fixed) return fixed < o.fixed;
+        return members < o.members;
+    }
+};
+static std::mutex g_tune_mu;
+static std::map<TuneKey, std::pair<int, int>> g_tune_cache;
+
+static bool tuned_choice_applies(const HvGroup &g, const mcpi_ham *hams, int mode, int sched)
+{
+    const bool full = g.basis->maps == MCPI_MAPS_FULL;
+    switch (mode) {
+    case 1: return sched >= 2 && sched <= 4 && (sched != 3 || full) && (sched != 4 || g.multi_ok);
+    case 2: return g.multi_ok && sched >= 2 && sched <= 4 && (sched != 2 || full);
+    case 3: return pairz_applicable(g.basis->d, hams[g.members[0]]->c);
+    case 4: return g.pzmv_ok;
+    default: return false;
+    }
+}
+
+static mcpi_status tune_group_cached(HvGroup &g, const mcpi_ham *hams, const std::vector<LzInst> &hin, int fixed,
+                                     hipStream_t s)
+{
+    if (g.basis->sectored()) return tune_group(g, hams, hin, fixed, s);
+    TuneKey key{g.basis, {}, g_batched_kernel.load(), fixed};
+    for (int x : g.members) {
+        key.members.push_back((uintptr_t)hams[x]);
+        key.members.push_back((uintptr_t)hams[x]->schedule);
+        key.members.push_back((uintptr_t)hams[x]->skip_zero);
+    }
+    {
+        std::lock_guard<std::mutex> lock(g_tune_mu);
+        auto it = g_tune_cache.find(key);
+        if (it != g_tune_cache.end() && tuned_choice_applies(g, hams, it->second.first, it->second.second)) {
+            g.mode = it->second.first;
+            g.sched = it->second.second;
+            return MCPI_OK;
+        }
+    }
+    MCPI_CHECK(tune_group(g, hams, hin, fixed, s));
+    std::lock_guard<std::mutex> lock(g_tune_mu);
+    if (g_tune_cache.size() >= 256) g_tune_cache.clear();
+    g_tune_cache[key] = {g.mode, g.sched};
+    return MCPI_OK;
+}
+
 // K instances of length lm.  lvec_ext[x] (may be null, or the array null):
 // caller's lm*n basis buffer.  diag_h/offd_h: host, K*lm (row x = instance
 // x).  Cout (may be null, or entries null): Ritz vectors; E0 (may be null
@@ -1139,7 +1200,7 @@ static mcpi_status lanczos_core(int K, const mcpi_ham *hams, const cplx *const *
     for (HvGroup &g : groups) {
         if (!g.batched) continue;
         const mcpi_ham h0 = hams[g.members[0]];
-        MCPI_CHECK(tune_group(g, hams, hin, (h0->schedule >= 2 && h0->schedule <= 4) ? h0->schedule : 0, s));
+        MCPI_CHECK(tune_group_cached(g, hams, hin, (h0->schedule >= 2 && h0->schedule <= 4) ? h0->schedule : 0, s));
     }
     for (const HvGroup &g : groups) MCPI_CHECK(launch_group(g, hams, hin, s));
     hipLaunchKernelGGL(lz_dot, gv, blk, 0, s, (const LzInst *)din);

@@ -202,3 +202,29 @@ def test_factorized_pair_form_several_members(oracle_lib):
         E_o, psi_o, _ = oracle.Basis(N, M).ground(lm, starts[x].numpy(), *mats[x])
         assert abs(E[x] - E_o) < 1e-10, (x, E[x], E_o)
         assert abs(abs(np.vdot(psi_o, outs[x].numpy())) - 1) < 1e-8
+
+
+def test_batched_repeat_calls_reuse_tuning():
+    """A second call on the same group reuses the first call's H.v kernel
+    choice (same kernels: identical results); after the handles are closed,
+    new Hamiltonians of another kind on the same basis (their handles may
+    reuse the old addresses) still get a choice that applies to them:
+    every E0 equals the one-instance solver's."""
+    b = mcpi_amd.Basis(8, 11, "full")
+    lm = 40
+    starts = [mcpi_amd.DeviceArray.from_numpy(_start(b.nc, 500 + x)) for x in range(5)]
+    hams = [mcpi_amd.Hamiltonian(b, oracle.ll_ho(11), oracle.ll_hint(11, 0.5 + 3 * x)) for x in range(5)]
+    note("batched repeat calls")
+    E1, _ = mcpi_amd.lanczos_batched(hams, starts, lm)
+    E2, _ = mcpi_amd.lanczos_batched(hams, starts, lm)
+    assert np.array_equal(E1, E2)
+    for H in hams:
+        H.close()
+    hams = []
+    for x in range(5):
+        _, Ho, Hint = oracle.random_inputs(8, 11, seed=600 + x)
+        hams.append(mcpi_amd.Hamiltonian(b, Ho, Hint))
+    E3, _ = mcpi_amd.lanczos_batched(hams, starts, lm)
+    for x in range(5):
+        E_single = hams[x].ground(lm, starts[x])[0]
+        assert abs(E3[x] - E_single) < 1e-10, (x, E3[x], E_single)
