@@ -392,7 +392,10 @@ mcpi_status mcpi_hip_tridiag_lowest(int n, const double *d, const double *e,
  * Ho/Hint, N, M) and normalized device start vectors; lm steps each, all
  * instances advanced together, one iteration replayed as one hipGraph.
  * Out: E0[n], steps[n] (host); C_out_dev (may be NULL, or hold NULL
- * entries): device Ritz vectors, nc each (may alias the start vectors). */
+ * entries): device Ritz vectors, nc each (may alias the start vectors).
+ * The H.v kernel of each group of instances sharing a basis is chosen by
+ * timing on the first call and kept for later calls with the same handles
+ * and settings. */
 mcpi_status mcpi_hip_lanczos_batched(int n, const mcpi_ham *hams,
                                      const mcpi_cplx *const *C0_dev, int lm,
                                      double *E0, int *steps,
