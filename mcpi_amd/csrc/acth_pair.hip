@@ -227,11 +227,13 @@ struct PairAmpLds {
 // R blocks of the factorized pair form (W = zb; a wavefront per block and
 // 64 rows); Z has R rows of ld entries.  MV > 1 (FAC only): the same for MV
 // members that share zb and the blocks, their C vectors mC and their Z
-// tables mY entries apart (the unranking and the targets are shared).
+// tables mY entries apart (the unranking and the targets are shared); grid
+// row y serves members y MV .. y MV + MV - 1.
 template <int M, bool FAC, int MV = 1>
 __global__ __launch_bounds__(PT) void k_pair_amp(PairArgs a)
 {
     static_assert(MV == 1 || FAC, "several members: factorized form only");
+    const int64_t m0 = MV > 1 ? (int64_t)blockIdx.y * MV : 0;   // first member of this grid row
     constexpr int NP = pair_count(M), U = PairAmpLds<M>::U;
     extern __shared__ __align__(16) unsigned char lds[];
     int64_t *tt = (int64_t *)lds;                              // [U]
@@ -324,7 +326,7 @@ __global__ __launch_bounds__(PT) void k_pair_amp(PairArgs a)
                     for (int m = 0; m < MV; m++)
 #pragma unroll
                         for (int k = 0; k < GBK; k++)
-                            v[m][k] = tg[k] >= 0 ? ldcp(a.C + m * a.mC + tg[k]) : cmake(0, 0);
+                            v[m][k] = tg[k] >= 0 ? ldcp(a.C + (m0 + m) * a.mC + tg[k]) : cmake(0, 0);
 #pragma unroll
                     for (int m = 0; m < MV; m++)
 #pragma unroll
@@ -335,7 +337,7 @@ __global__ __launch_bounds__(PT) void k_pair_amp(PairArgs a)
                     double2 o;
                     o.x = acc[m].re;
                     o.y = acc[m].im;
-                    *reinterpret_cast<double2 *>(a.Y + m * a.mY + (int64_t)r * a.ld + u0 + ul) = o;
+                    *reinterpret_cast<double2 *>(a.Y + (m0 + m) * a.mY + (int64_t)r * a.ld + u0 + ul) = o;
                 }
             }
             continue;
@@ -670,8 +672,9 @@ __global__ __launch_bounds__(PT) void k_pair_out(int N, int64_t nc, const int32_
 // basis, zb and the blocks (a group of Lieb-Liniger couplings): the row's
 // occupations, prefix sums, pair offsets and factors are computed once and
 // serve every member; member m reads its C, Z, za and h1 at C + m mC,
-// Z + m mZ and (bytes) za + m mK, h1 + m mK, and writes out + m mC.
-// Batches of BT terms for all members (MV BT loads in flight).
+// Z + m mZ and (bytes) za + m mK, h1 + m mK, and writes out + m mC; grid
+// row y serves members y MV .. y MV + MV - 1.  Batches of BT terms for all
+// members (MV BT loads in flight).
 template <int M, int MV>
 __global__ __launch_bounds__(PT) void k_pairz_out_mv(int N, int64_t nc, const int32_t *__restrict__ IF,
                                                      const int32_t *__restrict__ Bg, int64_t nu2, int64_t ld2,
@@ -689,6 +692,14 @@ __global__ __launch_bounds__(PT) void k_pairz_out_mv(int N, int64_t nc, const in
     __syncthreads();
     const int64_t i = (int64_t)blockIdx.x * PT + threadIdx.x;
     if (i >= nc) return;
+    {   // this grid row's first member
+        const int64_t m0 = (int64_t)blockIdx.y * MV;
+        C += m0 * mC;
+        out += m0 * mC;
+        Z += m0 * mZ;
+        za = (const cplx *)((const char *)za + m0 * mK);
+        h1 = (const cplx *)((const char *)h1 + m0 * mK);
+    }
     int v[M];
 #pragma unroll
     for (int k = 0; k < M; k++) v[k] = IF[i * M + k];
@@ -873,8 +884,8 @@ mcpi_status launch_pairz_m(const BasisDev &b, const HamCoefDev &c, const cplx *C
 // ws + m * R * ld2; vectors mC entries and coefficient blocks mK bytes
 // apart, member 0's tables in c)
 template <int M, int MV>
-mcpi_status launch_pairz_mv_m(const BasisDev &b, const HamCoefDev &c, const cplx *C, cplx *out, int64_t mC,
-                              int64_t mK, cplx *ws, hipStream_t st)
+mcpi_status launch_pairz_mv_m(const BasisDev &b, const HamCoefDev &c, int chunks, const cplx *C, cplx *out,
+                              int64_t mC, int64_t mK, cplx *ws, hipStream_t st)
 {
     int dev = 0, ncu = 0;
     MCPI_HIP(hipGetDevice(&dev));
@@ -888,10 +899,11 @@ mcpi_status launch_pairz_mv_m(const BasisDev &b, const HamCoefDev &c, const cplx
     int per_cu = 0;
     MCPI_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, (const void *)k_pair_amp<M, true, MV>, PT, sm));
     const int64_t nb = std::max<int64_t>(1, std::min<int64_t>(a.nut, (int64_t)ncu * std::max(per_cu, 1)));
-    hipLaunchKernelGGL((k_pair_amp<M, true, MV>), dim3((unsigned)nb), dim3(PT), sm, st, a);
+    hipLaunchKernelGGL((k_pair_amp<M, true, MV>), dim3((unsigned)nb, (unsigned)chunks), dim3(PT), sm, st, a);
     MCPI_HIP(hipGetLastError());
     const size_t so = sizeof(double) * (N + 3) + sizeof(int32_t) * M * (N + 1);
-    hipLaunchKernelGGL((k_pairz_out_mv<M, MV>), dim3((unsigned)((b.nc + PT - 1) / PT)), dim3(PT), so, st, N, b.nc,
+    hipLaunchKernelGGL((k_pairz_out_mv<M, MV>), dim3((unsigned)((b.nc + PT - 1) / PT), (unsigned)chunks), dim3(PT), so,
+                       st, N, b.nc,
                        (const int32_t *)b.IF, (const int32_t *)b.B, nu2, ld2, (const cplx *)ws, mZ,
                        (const cplx *)c.pza, (const int32_t *)c.pzblk, (const cplx *)c.ph1, mK, C, out, mC);
     MCPI_HIP(hipGetLastError());
@@ -975,15 +987,15 @@ mcpi_status launch_pairz(const BasisDev &b, const HamCoefDev &c, const cplx *C, 
     }
 }
 
-mcpi_status launch_pairz_multi(const BasisDev &b, const HamCoefDev &c, int mv, const cplx *C, cplx *out, int64_t mC,
-                               int64_t mK, cplx *ws, hipStream_t st)
+mcpi_status launch_pairz_multi(const BasisDev &b, const HamCoefDev &c, int mv, int chunks, const cplx *C, cplx *out,
+                               int64_t mC, int64_t mK, cplx *ws, hipStream_t st)
 {
     MCPI_ARG(pairz_applicable(b, c) && b.M <= MCPI_PAIRZ_MULTI_MAXM && (mv == 2 || mv == 4),
              "the factorized pair form for several members does not apply");
-    MCPI_ARG(mK % 16 == 0 && mC >= b.nc, "member strides");
+    MCPI_ARG(mK % 16 == 0 && mC >= b.nc && chunks >= 1 && chunks <= 65535, "member strides and chunks");
 #define MCPI_PZMV(MM) \
-    case MM: return mv == 4 ? launch_pairz_mv_m<MM, 4>(b, c, C, out, mC, mK, ws, st) \
-                            : launch_pairz_mv_m<MM, 2>(b, c, C, out, mC, mK, ws, st);
+    case MM: return mv == 4 ? launch_pairz_mv_m<MM, 4>(b, c, chunks, C, out, mC, mK, ws, st) \
+                            : launch_pairz_mv_m<MM, 2>(b, c, chunks, C, out, mC, mK, ws, st);
     switch (b.M) {
         MCPI_PZMV(3) MCPI_PZMV(4) MCPI_PZMV(5) MCPI_PZMV(6) MCPI_PZMV(7) MCPI_PZMV(8) MCPI_PZMV(9) MCPI_PZMV(10)
         MCPI_PZMV(11) MCPI_PZMV(12) MCPI_PZMV(13)

@@ -827,10 +827,12 @@ static bool pairz_multi_ok(const HvGroup &g, const mcpi_ham *hams)
     return true;
 }
 
+// Z tables of every member in a four-member chunk (at least four)
 static size_t pairz_multi_bytes(const HvGroup &g, const mcpi_ham *hams)
 {
     const BasisDev &b = g.basis->d;
-    return sizeof(cplx) * 4 * (size_t)hams[g.members[0]]->c.pzR * (size_t)pair_ld(nc_of(b.N - 2, b.M));
+    const size_t zm = std::max<size_t>(4, g.members.size() / 4 * 4);
+    return sizeof(cplx) * zm * (size_t)hams[g.members[0]]->c.pzR * (size_t)pair_ld(nc_of(b.N - 2, b.M));
 }
 
 static mcpi_status launch_group(const HvGroup &g, const mcpi_ham *hams, const std::vector<LzInst> &hin,
@@ -844,20 +846,23 @@ static mcpi_status launch_group(const HvGroup &g, const mcpi_ham *hams, const st
         return MCPI_OK;
     }
     if (g.batched && g.mode == 4) {
-        // the factorized pair form four members at a time (then two, then
-        // one) from the group's copies of the coefficient blocks
+        // the factorized pair form four members at a time, every chunk of
+        // four in one launch pair (then two, then one), from the group's
+        // copies of the coefficient blocks
         const BasisDev &bd = g.basis->d;
         const int m = (int)g.members.size();
         int k = 0;
         while (k < m) {
             const int mv = m - k >= 4 ? 4 : (m - k >= 2 ? 2 : 1);
+            const int chunks = mv == 4 ? (m - k) / 4 : 1;
             const HamCoefDev ck = shift_coef(g.c0, (ptrdiff_t)(k * g.cbytes));
             const size_t o = (size_t)k * (size_t)bd.nc;
             if (mv == 1)
                 MCPI_CHECK(launch_pairz(bd, ck, g.vin + o, g.HC + o, g.zws, s));
             else
-                MCPI_CHECK(launch_pairz_multi(bd, ck, mv, g.vin + o, g.HC + o, bd.nc, (int64_t)g.cbytes, g.zws, s));
-            k += mv;
+                MCPI_CHECK(launch_pairz_multi(bd, ck, mv, chunks, g.vin + o, g.HC + o, bd.nc, (int64_t)g.cbytes,
+                                              g.zws, s));
+            k += mv * chunks;
         }
         return MCPI_OK;
     }

@@ -377,13 +377,14 @@ mcpi_status launch_pair(const BasisDev &b, const HamCoefDev &c, const cplx *C, c
 bool pairz_applicable(const BasisDev &b, const HamCoefDev &c);
 mcpi_status launch_pairz(const BasisDev &b, const HamCoefDev &c, const cplx *C, cplx *out, cplx *ws,
                          hipStream_t st);
-// the same for mv (2 or 4) Hamiltonians of one basis whose pair matrices
-// share the blocks and b factors (M <= MCPI_PAIRZ_MULTI_MAXM): member m's
-// vectors at C + m mC and out + m mC, its coefficient block mK bytes after
-// member m - 1's (member 0's tables in c); a workspace of
-// mv x (blocks) x pair_ld(NC(N-2, M)) complex entries
-mcpi_status launch_pairz_multi(const BasisDev &b, const HamCoefDev &c, int mv, const cplx *C, cplx *out, int64_t mC,
-                               int64_t mK, cplx *ws, hipStream_t st);
+// the same for chunks x mv (mv 2 or 4) Hamiltonians of one basis whose
+// pair matrices share the blocks and b factors (M <=
+// MCPI_PAIRZ_MULTI_MAXM), mv members per thread: member m's vectors at
+// C + m mC and out + m mC, its coefficient block mK bytes after member
+// m - 1's (member 0's tables in c); a workspace of chunks x mv x (blocks)
+// x pair_ld(NC(N-2, M)) complex entries
+mcpi_status launch_pairz_multi(const BasisDev &b, const HamCoefDev &c, int mv, int chunks, const cplx *C, cplx *out,
+                               int64_t mC, int64_t mK, cplx *ws, hipStream_t st);
 // build the requested maps of a basis whose IF table is in place
 mcpi_status basis_build_parts(mcpi_basis b, int want_map, int want_ot, int want_tt);
 
