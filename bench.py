@@ -374,11 +374,28 @@ def batched_lanczos(n_inst: int = 64, lm: int = 120, seed: int = 64120, serial: 
     t_batched = time.perf_counter() - t
     E_s = np.full(n_inst, np.nan)
     t_serial = float("nan")
+    t_each = np.zeros(n_inst)
     if serial:
         t = time.perf_counter()
         for x in range(n_inst):
+            t1 = time.perf_counter()
             E_s[x] = hams[x].ground(lm, starts[x], out=outs[x])[0]
+            t_each[x] = time.perf_counter() - t1
         t_serial = time.perf_counter() - t
+    # the same split by basis: each basis's instances batched alone against
+    # their share of the one-by-one solves
+    per_shape = {}
+    if serial:
+        for k in sorted(set(int(p) for p in picks)):
+            xs = [x for x in range(n_inst) if picks[x] == k]
+            mcpi_amd.synchronize()
+            t = time.perf_counter()
+            mcpi_amd.lanczos_batched([hams[x] for x in xs], [starts[x] for x in xs], lm, [outs[x] for x in xs])
+            tb = time.perf_counter() - t
+            ts = float(t_each[xs].sum())
+            per_shape[f"({shapes[k][0]},{shapes[k][1]})"] = {"instances": len(xs), "s_batched": round(tb, 4),
+                                                            "s_per_instance_total": round(ts, 4),
+                                                            "speedup": round(ts / tb, 3)}
     dE = float(np.abs(E_b - E_s).max()) if serial else None
     counts = {f"({shapes[k][0]},{shapes[k][1]})": int((picks == k).sum()) for k in range(3)}
     cpu = None
@@ -426,6 +443,7 @@ def batched_lanczos(n_inst: int = 64, lm: int = 120, seed: int = 64120, serial: 
         "s_batched": round(t_batched, 4),
         "s_per_instance_total": round(t_serial, 4),
         "max_abs_dE0_batched_vs_per_instance": dE if serial else None,
+        "per_basis": per_shape or None,
         "all_steps_full": bool((steps_b == lm).all()),
         "basis_build_s_excluded": round(build_s, 3),
         "cpu_oracle_sample": cpu,
