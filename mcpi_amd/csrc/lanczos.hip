@@ -354,11 +354,15 @@ __global__ __launch_bounds__(LZ_T) void lz_gemv_dots(const LanczosState *sts, co
     for (int tt = t0; tt < t1; tt++) {
         const LZ_GLOBAL cplx *vt = gp(I.lvec) + (int64_t)tt * n;
         double re = 0, im = 0;
+        // small chains (TG = 8): rows past n load row n - 1 and meet h = 0,
+        // with no branch, so all LZ_R loads are in flight together; large
+        // ones keep the branch (the loads then wait one by one, which with
+        // many resident waves measured faster: 13.0 M rows, 200 steps)
 #pragma unroll
         for (int r = 0; r < LZ_R; r++) {
-            int64_t j = base + (int64_t)r * LZ_T;
-            if (j < n) {
-                cplx x = ldv_nt(vt + j);
+            const int64_t j = base + (int64_t)r * LZ_T;
+            if (TG == 8 || j < n) {
+                const cplx x = ldv_nt(vt + (j < n ? j : n - 1));
                 re += x.re * h[r].re + x.im * h[r].im;
                 im += x.re * h[r].im - x.im * h[r].re;
             }
@@ -412,8 +416,11 @@ __global__ __launch_bounds__(LZ_T) void lz_fin_ortho(const LanczosState *sts, co
 
 // 8: HC -= sum_t v_t o_t;  partial ||HC||^2 for the next beta.  A thread
 // owns RR rows of each block step (rows j + r * LZ_T) and walks the basis
-// vectors with one load per row in flight, two vectors at a time, so every
-// lane keeps 2 * RR independent 16-byte loads outstanding.  The o_t are the
+// vectors two at a time.  With RR = 2 every row's load is issued
+// unconditionally, so a lane keeps 2 * RR independent 16-byte loads
+// outstanding (a branch per row made the compiler wait for each load
+// before the next: 4.6 against 6.3 TB/s at 352,716 rows, where few waves
+// are resident).  The o_t are the
 // same for every lane: they are read as uniform (scalar-cache) loads.  RR
 // is 4, or 2 when the chain's instances have too few rows for 4 to fill
 // the device (update_rows).
@@ -437,6 +444,9 @@ __global__ __launch_bounds__(LZ_T) void lz_reorth_norm(const LanczosState *sts, 
             const int64_t j = base + (int64_t)r * LZ_T;
             h[r] = (j < n) ? ldv(HC + j) : cmake(0, 0);
         }
+        // RR = 2 (small chains): rows past n load row n - 1, never stored,
+        // with no branch, so the loads of both unrolled vectors are in flight
+        // together; RR = 4 keeps the branch (faster with many resident waves)
 #pragma unroll 2
         for (int tt = 0; tt < nt; tt++) {
             const cplx ot = cmake(o[tt].re, o[tt].im);
@@ -444,8 +454,8 @@ __global__ __launch_bounds__(LZ_T) void lz_reorth_norm(const LanczosState *sts, 
 #pragma unroll
             for (int r = 0; r < RR; r++) {
                 const int64_t j = base + (int64_t)r * LZ_T;
-                if (j < n) {
-                    const cplx x = ldv_nt(vt + j);
+                if (RR == 2 || j < n) {
+                    const cplx x = ldv_nt(vt + (j < n ? j : n - 1));
                     h[r].re -= x.re * ot.re - x.im * ot.im;
                     h[r].im -= x.re * ot.im + x.im * ot.re;
                 }
@@ -488,13 +498,11 @@ __global__ __launch_bounds__(LZ_T) void lz_reorth_part(const LanczosState *sts, 
         const cplx ot = cmake(o[tt].re, o[tt].im);
         const LZ_GLOBAL cplx *vt = gp(I.lvec) + (int64_t)tt * n;
 #pragma unroll
-        for (int r = 0; r < LZ_R; r++) {
+        for (int r = 0; r < LZ_R; r++) {   // rows past n: row n - 1, never stored
             const int64_t j = base + (int64_t)r * LZ_T;
-            if (j < n) {
-                const cplx x = ldv_nt(vt + j);
-                p[r].re += x.re * ot.re - x.im * ot.im;
-                p[r].im += x.re * ot.im + x.im * ot.re;
-            }
+            const cplx x = ldv_nt(vt + (j < n ? j : n - 1));
+            p[r].re += x.re * ot.re - x.im * ot.im;
+            p[r].im += x.re * ot.im + x.im * ot.re;
         }
     }
     LZ_GLOBAL cplx *dst = gp(I.rpart) + (int64_t)blockIdx.y * n;
